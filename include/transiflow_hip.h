@@ -1,0 +1,103 @@
+/*
+ * C ABI of libtransiflow_hip.so -- the MI355X (gfx950) backend of TransiFlow's
+ * continuation hot path.
+ *
+ * The reference has no FFI: its backend boundary is the Python class
+ * transiflow/interface/SciPy.py (contract: transiflow/interface/BaseInterface.py).
+ * transiflow_amd/interface.py presents that Python contract and calls the
+ * functions below through ctypes.  Each entry point names the reference
+ * function it replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative value on
+ * failure (the message is then available from tf_last_error()).  All array
+ * arguments are device pointers (HBM) unless named *_host; ``stream`` is a
+ * hipStream_t passed as void*.  Row/column indices are int32, values FP64
+ * (FP32 for the *_f32 solver variants).  No function allocates or frees
+ * caller memory.
+ */
+#ifndef TRANSIFLOW_HIP_H
+#define TRANSIFLOW_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tf_problem tf_problem;
+
+/* Message of the last failed call on this thread. */
+const char* tf_last_error(void);
+
+/* Devices visible to the library's HIP runtime (-1 on error).  No reference
+ * counterpart; lets the host layer fail loudly without a GPU. */
+int tf_device_count(void);
+
+/* ---- problem (replaces Discretization.__init__, Discretization.py:106-143) ---- */
+
+/* Grid nx*ny*nz with dim (2|3) and dof (dim+1..5); x, y, z are host
+ * coordinate vectors of length n+3 (utils.create_uniform_coordinate_vector,
+ * utils.py:175-197); *_periodic as Discretization.py:123-128. */
+int tf_problem_create(tf_problem** out, int nx, int ny, int nz, int dim, int dof,
+                      const double* x_host, const double* y_host, const double* z_host,
+                      int x_periodic, int y_periodic, int z_periodic);
+int tf_problem_destroy(tf_problem* p);
+int tf_problem_rows(const tf_problem* p);
+
+/* Linear-part scalars (Discretization._linear_part_2D/3D, Discretization.py:229-317):
+ * lin_scale = 1/(Re sqrt(Gr)), T_scale = 1/(Pr sqrt(Gr)), S_scale = 1/(Le Pr sqrt(Gr)),
+ * rbp_coef = Bi/(Bi+1) when rbp; nonlinear = 0 when Re == 0 and dof <= dim+1
+ * (Discretization.py:333-335). */
+int tf_problem_set_physics(tf_problem* p, double lin_scale, double T_scale, double S_scale,
+                           double rbp_coef, int rbp, int nonlinear);
+
+/* Boundary program: the BoundaryConditions primitives in call order
+ * (Discretization.boundaries, Discretization.py:704-714).  kind 0 = wall
+ * (no_slip sign -1 / free_slip sign +1, BoundaryConditions.py:55-233),
+ * kind 1 = _constant_forcing on variable var with constants fc, ac
+ * (BoundaryConditions.py:471-544).  side +1 = east/north/top.  Host arrays. */
+int tf_problem_set_bcs(tf_problem* p, int nops, const int* kind_host, const int* axis_host,
+                       const int* side_host, const int* var_host, const double* sign_host,
+                       const double* fc_host, const double* ac_host);
+
+/* ---- assembly ---- */
+
+/* F(state) (Discretization.rhs, Discretization.py:367-390; SciPy.Interface.rhs, SciPy.py:40-41). */
+int tf_rhs(tf_problem* p, const double* state, double* F, void* stream);
+
+/* Jacobian in CSR (Discretization.jacobian + assemble_jacobian + CrsMatrix.compress,
+ * Discretization.py:392-415,474-524, CrsMatrix.py:52-71; SciPy.Interface.jacobian, SciPy.py:43-45).
+ * count: row_ptr[0..n] is filled by the fill pass except row_ptr[n] = nnz,
+ * which count writes; nnz_host (optional) receives nnz after a stream sync.
+ * fill: writes row_ptr[0..n-1], col[0..nnz), val[0..nnz). */
+int tf_jacobian_count(tf_problem* p, const double* state, int* row_ptr, int* nnz_host, void* stream);
+int tf_jacobian_fill(tf_problem* p, const double* state, int* row_ptr, int* col, double* val,
+                     void* stream);
+
+/* Diagonal mass matrix (Discretization.mass_matrix, Discretization.py:417-437;
+ * SciPy.Interface.mass_matrix, SciPy.py:47-49), same count/fill protocol. */
+int tf_mass_count(tf_problem* p, int* row_ptr, int* nnz_host, void* stream);
+int tf_mass_fill(tf_problem* p, int* row_ptr, int* col, double* val, void* stream);
+
+/* ---- vectors (numpy arithmetic and utils.dot/norm, utils.py:5-21) ---- */
+
+int tf_axpby_f64(long n, double alpha, const double* x, double beta, const double* y, double* out,
+                 void* stream);
+int tf_scale_f64(long n, double alpha, const double* x, double* out, void* stream);
+int tf_div_f64(long n, const double* x, double a, double* out, void* stream);
+int tf_f64_to_f32(long n, const double* x, float* out, void* stream);
+int tf_f32_to_f64(long n, const float* x, double* out, void* stream);
+int tf_reduce_work_size(void);
+int tf_dot_f64(long n, const double* x, const double* y, double* work, double* out_dev,
+               double* out_host, int norm, void* stream);
+
+/* y = A x; gauge >= 0 applies the pressure gauge of SciPy.Interface.compute_bordered_matrix
+ * (SciPy.py:95-103): column gauge removed, row gauge -> -x[gauge]. */
+int tf_spmv_f64(int n, const int* row_ptr, const int* col, const double* val, const double* x,
+                double* y, int gauge, void* stream);
+int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, const float* x,
+                float* y, int gauge, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRANSIFLOW_HIP_H */

@@ -1,0 +1,91 @@
+"""ctypes binding of ``libtransiflow_hip.so`` (C ABI in include/transiflow_hip.h).
+
+The library is built in-tree by ``python -m transiflow_amd.build``.  There is
+no fallback: if the library is missing every backend call raises.
+"""
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libtransiflow_hip.so')
+
+_c_int = ctypes.c_int
+_c_long = ctypes.c_long
+_c_double = ctypes.c_double
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    'tf_last_error': (ctypes.c_char_p, []),
+    'tf_device_count': (_c_int, []),
+    'tf_problem_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int,
+                                   _dp, _dp, _dp, _c_int, _c_int, _c_int]),
+    'tf_problem_destroy': (_c_int, [_vp]),
+    'tf_problem_set_physics': (_c_int, [_vp, _c_double, _c_double, _c_double, _c_double, _c_int, _c_int]),
+    'tf_problem_set_bcs': (_c_int, [_vp, _c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
+    'tf_problem_rows': (_c_int, [_vp]),
+    'tf_rhs': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_jacobian_count': (_c_int, [_vp, _vp, _vp, _ip, _vp]),
+    'tf_jacobian_fill': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    'tf_mass_count': (_c_int, [_vp, _vp, _ip, _vp]),
+    'tf_mass_fill': (_c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'tf_axpby_f64': (_c_int, [_c_long, _c_double, _vp, _c_double, _vp, _vp, _vp]),
+    'tf_scale_f64': (_c_int, [_c_long, _c_double, _vp, _vp, _vp]),
+    'tf_div_f64': (_c_int, [_c_long, _vp, _c_double, _vp, _vp]),
+    'tf_f64_to_f32': (_c_int, [_c_long, _vp, _vp, _vp]),
+    'tf_f32_to_f64': (_c_int, [_c_long, _vp, _vp, _vp]),
+    'tf_reduce_work_size': (_c_int, []),
+    'tf_dot_f64': (_c_int, [_c_long, _vp, _vp, _vp, _vp, _dp, _c_int, _vp]),
+    'tf_spmv_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_spmv_f32': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
+}
+
+_lib = None
+
+
+class BackendError(RuntimeError):
+    """A HIP backend call failed; the message is the library's last error."""
+
+
+def load():
+    """Load the shared library and declare every exported signature.
+
+    PyTorch is imported first: it carries its own libamdhip64.so.7 (same
+    SONAME as /opt/rocm's), and loading it before our library makes the
+    dynamic linker bind our library to that one HIP runtime instead of
+    bringing a second runtime into the process.
+    """
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BackendError('HIP backend library not built (%s); run `python -m transiflow_amd.build`'
+                           % LIB_PATH)
+    import torch  # noqa: F401  (see docstring)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call ``name`` and raise BackendError on a non-zero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        raise BackendError('%s failed (%d): %s' % (name, rc, lib.tf_last_error().decode()))
+    return rc
+
+
+def dptr(arr):
+    """ctypes double* of a contiguous numpy float64 array."""
+    return arr.ctypes.data_as(_dp)
+
+
+def iptr(arr):
+    return arr.ctypes.data_as(_ip)

@@ -1,0 +1,168 @@
+// Vector kernels and CSR SpMV for the FP64 / FP32 solve path.
+//
+// Reductions are deterministic: a fixed grid of TF_RED_BLOCKS workgroups each
+// reduces a fixed contiguous slice in a fixed tree order, then one workgroup
+// reduces the partials; the same inputs give the same bits on every run.
+#include "tf_common.h"
+
+#include <type_traits>
+
+namespace {
+
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+
+// out may alias x or y (element-wise read-then-write by the same thread)
+template <typename T>
+__global__ __launch_bounds__(256) void k_axpby(long n, double alpha, const T* x, double beta, const T* y, T* out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (T)(alpha * (double)x[i] + beta * (double)y[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale(long n, double alpha, const T* x, T* out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (T)(alpha * (double)x[i]);
+}
+
+// true division (numpy's x / a rounds differently from x * (1/a))
+__global__ __launch_bounds__(256) void k_div(long n, const double* x, double a, double* out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = x[i] / a;
+}
+
+template <typename Tin, typename Tout>
+__global__ __launch_bounds__(256) void k_convert(long n, const Tin* __restrict__ x, Tout* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (Tout)x[i];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_dot_partial(long n, const T* __restrict__ x, const T* __restrict__ y,
+                                                             double* __restrict__ partial) {
+  __shared__ double sh[RED_THREADS];
+  const long chunk = (n + gridDim.x - 1) / gridDim.x;
+  const long b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+  double acc = 0.0;
+  for (long i = b0 + threadIdx.x; i < b1; i += RED_THREADS) acc += (double)x[i] * (double)y[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partial, int np,
+                                                              double* __restrict__ out, int sqrt_abs) {
+  __shared__ double sh[RED_THREADS];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += RED_THREADS) acc += partial[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) *out = sqrt_abs ? sqrt(fabs(s)) : s;
+}
+
+// y = A x for CSR A; with gauge >= 0 the operator is the pressure-gauged
+// system of the SciPy backend (SciPy.py:95-103): column ``gauge`` is removed
+// and row ``gauge`` becomes -x[gauge].
+template <typename TV, typename TX>
+__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                              const TV* __restrict__ v, const TX* __restrict__ x, TX* __restrict__ y,
+                                              int gauge) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  if (row == gauge) {
+    y[row] = -x[row];
+    return;
+  }
+  using Acc = typename std::conditional<sizeof(TX) == 8, double, float>::type;
+  Acc acc = 0;
+  const int e = rp[row + 1];
+  for (int k = rp[row]; k < e; ++k) {
+    const int c = ci[k];
+    if (c != gauge) acc += (Acc)v[k] * (Acc)x[c];
+  }
+  y[row] = (TX)acc;
+}
+
+int grid_for(long n) {
+  long g = (n + 255) / 256;
+  return (int)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tf_axpby_f64(long n, double alpha, const double* x, double beta, const double* y, double* out, void* stream) {
+  hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, beta, y, out);
+  TF_LAUNCH_CHECK("k_axpby");
+  return 0;
+}
+
+int tf_scale_f64(long n, double alpha, const double* x, double* out, void* stream) {
+  hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, out);
+  TF_LAUNCH_CHECK("k_scale");
+  return 0;
+}
+
+int tf_div_f64(long n, const double* x, double a, double* out, void* stream) {
+  hipLaunchKernelGGL(k_div, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, x, a, out);
+  TF_LAUNCH_CHECK("k_div");
+  return 0;
+}
+
+int tf_f64_to_f32(long n, const double* x, float* out, void* stream) {
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, x, out);
+  TF_LAUNCH_CHECK("k_convert");
+  return 0;
+}
+
+int tf_f32_to_f64(long n, const float* x, double* out, void* stream) {
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, x, out);
+  TF_LAUNCH_CHECK("k_convert");
+  return 0;
+}
+
+// Deterministic dot product (or 2-norm when norm != 0) into out_dev[0];
+// ``work`` holds at least tf_reduce_work_size() doubles.  When out_host is
+// not null the stream is synchronised and the value copied there.
+int tf_reduce_work_size(void) { return RED_BLOCKS; }
+
+int tf_dot_f64(long n, const double* x, const double* y, double* work, double* out_dev, double* out_host, int norm,
+               void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, x, y, work);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, st, work, RED_BLOCKS, out_dev, norm);
+  TF_LAUNCH_CHECK("tf_dot_f64");
+  if (out_host) {
+    TF_HIP(hipMemcpyAsync(out_host, out_dev, sizeof(double), hipMemcpyDeviceToHost, st));
+    TF_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const double* x, double* y, int gauge,
+                void* stream) {
+  hipLaunchKernelGGL((k_spmv<double, double>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
+                     x, y, gauge);
+  TF_LAUNCH_CHECK("k_spmv f64");
+  return 0;
+}
+
+int tf_spmv_f32(int n, const int* rp, const int* ci, const float* v, const float* x, float* y, int gauge,
+                void* stream) {
+  hipLaunchKernelGGL((k_spmv<float, float>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v, x,
+                     y, gauge);
+  TF_LAUNCH_CHECK("k_spmv f32");
+  return 0;
+}
+
+}  // extern "C"
