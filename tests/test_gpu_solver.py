@@ -1,0 +1,101 @@
+"""GPU linear solves and Newton/continuation parity.
+
+solve(): restarted GMRES on the pressure-gauged Jacobian, preconditioned by
+the exact block-tridiagonal line factorisation, must reach a relative
+residual of 1e-10 and agree with the oracle's sparse-LU (scipy splu) solution
+of the same gauged system (SciPy.py:204-258 semantics) to 1e-8 relative.
+Newton and the pseudo-arclength continuation are compared with vectors
+written from the reference (tests/golden/*.npz).
+"""
+import os
+
+import numpy
+import pytest
+
+from tests.helpers import seeded_state
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+pytestmark = pytest.mark.gpu
+
+
+def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64'):
+    from oracle.interface import Interface as OracleInterface
+    p = dict(params)
+    p['HIP Solver'] = {'Preconditioner': prec, 'Precision': precision}
+    itf = hip.Interface(p, *shape, dim, dof)
+    ref = OracleInterface(dict(params), *shape, dim, dof)
+    state = seeded_state(itf.n, seed)
+    rhs = seeded_state(itf.n, seed + 100)
+    J = itf.jacobian(state)
+    y = itf.array_from_vector(itf.solve(J, rhs))
+    yo = ref.solve(ref.jacobian(state), rhs)
+    err = numpy.linalg.norm(y - yo) / numpy.linalg.norm(yo)
+    # residual of the gauged system, computed on the host
+    A = ref.compute_bordered_matrix(ref.jacobian(state), fix_pressure_row=True)
+    b = rhs.copy()
+    b[ref.pressure_row] = 0
+    res = numpy.linalg.norm(A @ y - b) / numpy.linalg.norm(b)
+    its, rel, name, fresh = itf.solver.log[-1]
+    print('%s %s: GMRES its %d, rel res %.2e (host %.2e), |y - y_splu|/|y_splu| %.2e'
+          % (shape, name, its, rel, res, err), flush=True)
+    assert res <= 1e-9
+    assert err <= 1e-8
+    return itf
+
+
+@pytest.mark.parametrize('label,params,shape,dim,dof,seed', [
+    ('ldc2d_16', {'Reynolds Number': 100}, (16, 16, 1), 2, 3, 1),
+    ('ldc2d_64x40', {'Reynolds Number': 2000}, (64, 40, 1), 2, 3, 2),
+    ('ldc2d_stretched', {'Reynolds Number': 500, 'Grid Stretching': True}, (48, 48, 1), 2, 3, 3),
+    ('rb2d_64x16', {'Problem Type': 'Rayleigh-Benard', 'Prandtl Number': 1.0, 'Rayleigh Number': 3000.0,
+                    'Biot Number': 1.0, 'X-max': 4.0}, (64, 16, 1), 2, 4, 4),
+    ('ldc3d_6', {'Reynolds Number': 100}, (6, 6, 6), 3, 4, 5),
+])
+def test_solve_line(hip, label, params, shape, dim, dof, seed):
+    _solve_case(hip, params, shape, dim, dof, seed)
+
+
+def test_solve_full_size_ldc256(hip):
+    """C2 size: one Jacobian at a seeded state, Re = 2000."""
+    _solve_case(hip, {'Reynolds Number': 2000}, (256, 256, 1), 2, 3, 6)
+
+
+def test_solve_mixed_storage(hip):
+    """FP32-stored factor, FP64 GMRES: same 1e-10 residual bar."""
+    _solve_case(hip, {'Reynolds Number': 1000}, (64, 64, 1), 2, 3, 7, precision='mixed')
+
+
+def test_newton_ldc2d_16(hip):
+    """C1: Newton from zero at Re = 100 against the reference's result."""
+    g = numpy.load(os.path.join(GOLDEN, 'ldc2d_16.npz'))
+    itf = hip.Interface({'Reynolds Number': 100}, 16, 16, 1, 2, 3)
+    cont = hip.Continuation(itf, newton_tolerance=1e-8, printer=None)
+    x = cont.newton(itf.vector(), tol=1e-8)
+    xh = itf.array_from_vector(x)
+    err = numpy.linalg.norm(xh - g['x_newton']) / numpy.linalg.norm(g['x_newton'])
+    print('Newton 16x16: rel err vs reference %.2e' % err)
+    assert err <= 1e-8
+
+
+def test_continuation_ldc2d_8(hip):
+    """LDC 2D 8x8, Re 0 -> 2000, ds = 100: accepted mu sequence, Newton counts and
+    final state against the reference's trajectory."""
+    g = numpy.load(os.path.join(GOLDEN, 'cont_ldc2d_8.npz'))
+    itf = hip.Interface({}, 8, 8, 1, 2, 3)
+    cont = hip.Continuation(itf, newton_tolerance=1e-8, printer=None)
+    x0 = cont.newton(itf.vector())
+    mus, its = [], []
+
+    def cb(_itf, _x, mu):
+        mus.append(mu)
+        its.append(cont.newton_iterations)
+
+    x, mu = cont.continuation(x0, 'Reynolds Number', 0, 2000, 100, callback=cb)
+    assert len(mus) == len(g['mus'])
+    numpy.testing.assert_allclose(mus, g['mus'], rtol=1e-7, atol=1e-7)
+    assert list(its) == list(g['newton_its'])
+    xh = itf.array_from_vector(x)
+    err = numpy.linalg.norm(xh - g['x']) / numpy.linalg.norm(g['x'])
+    print('continuation 8x8: final mu %.6f, rel err of final state %.2e' % (mu, err))
+    assert abs(mu - float(g['mu'])) <= 1e-6
+    assert err <= 1e-8

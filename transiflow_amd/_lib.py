@@ -40,6 +40,25 @@ SIGNATURES = {
     'tf_dot_f64': (_c_int, [_c_long, _vp, _vp, _vp, _vp, _dp, _c_int, _vp]),
     'tf_spmv_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
     'tf_spmv_f32': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_set_entry_f64': (_c_int, [_vp, _c_long, _c_double, _vp]),
+    'tf_fill_f64': (_c_int, [_c_long, _c_double, _vp, _vp]),
+    'tf_krylov_work_size': (_c_int, []),
+    'tf_krylov_max_basis': (_c_int, []),
+    'tf_multidot_f64': (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_multidot_f32': (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_multiaxpy_f64': (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _c_double, _vp]),
+    'tf_multiaxpy_f32': (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _c_double, _vp]),
+    'tf_normalize_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    'tf_normalize_f32': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    'tf_cellblock_setup': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    'tf_cellblock_apply_f64': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'tf_cellblock_apply_f32': (_c_int, [_c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'tf_linelu_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
+    'tf_linelu_destroy': (_c_int, [_vp]),
+    'tf_linelu_factor': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_linelu_status': (_c_int, [_vp, _ip, _vp]),
+    'tf_linelu_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_linelu_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -37,6 +37,8 @@ COMMON = ['-O3', '--offload-arch=' + ARCH, '-fPIC', '-std=c++17', '-Wall', '-Wno
 SOURCES = {
     'assemble.hip': ['-ffp-contract=off', '-fno-signed-zeros', '-ffinite-math-only'],
     'blas.hip': [],
+    'krylov.hip': [],
+    'linelu.hip': [],
 }
 
 

@@ -92,6 +92,12 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
   y[row] = (TX)acc;
 }
 
+__global__ void k_set_entry(double* x, long i, double v) { x[i] = v; }
+
+__global__ __launch_bounds__(256) void k_fill(long n, double v, double* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
+}
+
 int grid_for(long n) {
   long g = (n + 255) / 256;
   return (int)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
@@ -100,6 +106,18 @@ int grid_for(long n) {
 }  // namespace
 
 extern "C" {
+
+int tf_set_entry_f64(double* x, long i, double v, void* stream) {
+  hipLaunchKernelGGL(k_set_entry, dim3(1), dim3(1), 0, (hipStream_t)stream, x, i, v);
+  TF_LAUNCH_CHECK("k_set_entry");
+  return 0;
+}
+
+int tf_fill_f64(long n, double v, double* x, void* stream) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, v, x);
+  TF_LAUNCH_CHECK("k_fill");
+  return 0;
+}
 
 int tf_axpby_f64(long n, double alpha, const double* x, double beta, const double* y, double* out, void* stream) {
   hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, beta, y, out);
