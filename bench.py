@@ -1,0 +1,263 @@
+"""Benchmark: pseudo-arclength continuation steps/s on one MI355X.
+
+Workload (default): lid-driven cavity 2D, 256 x 256 (n = 196,608 unknowns),
+FP64, Newton from zero at Re = 0, then pseudo-arclength continuation in the
+Reynolds number from 0 to 2000 with ds = 100 (ds_min 0.01, ds_max 1000) and
+Newton tolerance 1e-8 -- the reference's ``Continuation(interface,
+newton_tolerance=1e-8).continuation(x0, 'Reynolds Number', 0, 2000, 100)``.
+One "step" is one accepted pseudo-arclength step; the timed region is the
+whole continuation including the initial Newton solve and tangent, with the
+state resident in HBM.
+
+Variants: ``fp64`` (GMRES to 1e-10 with the exact FP64 line factorisation,
+refactored every Newton iteration) and ``mixed`` (FP32-stored factorisation
+reused across Newton iterations until GMRES needs more than a few iterations;
+GMRES, residuals and Newton in FP64).  ``value`` is the fastest variant that
+reproduces the FP64 variant's final state to 1e-8.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'continuation steps/s at 1e-8 residual: learned+mixed vs FP64 GPU; assembly GB/s'
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_continuation(tf, torch, n, precision, target, ds, maxit, restart=30):
+    params = {'HIP Solver': {'Precision': precision, 'Restart': restart}}
+    itf = tf.Interface(params, n, n)
+    cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x0 = cont.newton(itf.vector(), tol=1e-8)
+    newton0 = cont.newton_iterations
+    x, mu = cont.continuation(x0, 'Reynolds Number', 0, target, ds, maxit=maxit)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    acc = sum(1 for h in cont.history if h[0] == 'accepted')
+    iters = sum(h[1] for h in cont.history)
+    fact = sum(1 for entry in itf.solver.log if entry[3])
+    gm = sum(entry[0] for entry in itf.solver.log)
+    worst = max(entry[1] for entry in itf.solver.log)
+    return {
+        'itf': itf, 'x': x, 'mu': mu, 'seconds': dt, 'accepted': acc, 'corrector_iterations': iters,
+        'initial_newton_iterations': newton0, 'solves': len(itf.solver.log), 'factorizations': fact,
+        'gmres_iterations': gm, 'worst_relative_residual': worst,
+        'steps_per_s': acc / dt,
+    }
+
+
+def kernel_times(lib, ids):
+    out = {}
+    for name, pid in ids.items():
+        c = ctypes.c_long()
+        ms = ctypes.c_double()
+        lib.tf_prof_read(pid, ctypes.byref(c), ctypes.byref(ms))
+        out[name] = (c.value, ms.value)
+    return out
+
+
+def assembly_roofline(tf, torch, itf, x, reps=20):
+    """Event-timed count/fill/rhs kernels at state x; algorithmic bytes of one
+    J + F evaluation: 8n (state) + 12 nnz (values + col) + 4(n+1) (row_ptr) + 8n (F)."""
+    from transiflow_amd import _lib
+    lib = _lib.load()
+    lib.tf_prof_reset()
+    lib.tf_prof_enable(1)
+    for _ in range(reps):
+        F = itf.rhs(x)
+        J = itf.jacobian(x)
+    torch.cuda.synchronize()
+    lib.tf_prof_enable(0)
+    kt = kernel_times(lib, {'count': 0, 'fill': 1, 'rhs': 2})
+    n = itf.n
+    nnz = J.nnz
+    t = sum(ms for (_, ms) in kt.values()) / reps * 1e-3
+    bytes_alg = 8 * n + 12 * nnz + 4 * (n + 1) + 8 * n
+    fill_ms = kt['fill'][1] / kt['fill'][0]
+    fill_bytes = 8 * n + 12 * nnz + 4 * (n + 1)
+    del F
+    return {
+        'unit': 'one Jacobian (CSR) + rhs evaluation', 'n': n, 'nnz': nnz, 'bytes': bytes_alg,
+        'seconds': t, 'GBps': bytes_alg / t / 1e9, 'frac_hbm_peak': bytes_alg / t / 1e9 / HBM_PEAK_GBS,
+        'fill_kernel_us': fill_ms * 1e3, 'fill_kernel_GBps': fill_bytes / (fill_ms * 1e-3) / 1e9,
+        'count_kernel_us': kt['count'][1] / kt['count'][0] * 1e3, 'rhs_kernel_us': kt['rhs'][1] / kt['rhs'][0] * 1e3,
+    }
+
+
+def solver_kernels(tf, torch, itf, x):
+    """Event-timed kernels of one factorisation + solve (line preconditioner)."""
+    from transiflow_amd import _lib
+    lib = _lib.load()
+    lib.tf_prof_reset()
+    lib.tf_prof_enable(1)
+    J = itf.jacobian(x)
+    F = itf.rhs(x)
+    J.prec = None
+    itf.solver._lagged = None
+    y = itf.solve(J, F)
+    torch.cuda.synchronize()
+    lib.tf_prof_enable(0)
+    del y
+    return kernel_times(lib, {'gj_panel': 3, 'gj_update': 4, 'gemm': 5, 'gemv': 6, 'spmv': 7}), itf.solver.line_stats()
+
+
+def cpu_baseline(n, x_host, mu, iters_total, accepted):
+    """Time the oracle (numpy/scipy restatement of the SciPy backend) for one
+    Newton-corrector iteration at the final state and scale by the GPU run's
+    iteration count (same algorithm, same counts)."""
+    import numpy
+    from oracle.interface import Interface as OracleInterface
+    ref = OracleInterface({'Reynolds Number': mu}, n, n)
+    x = numpy.asarray(x_host)
+    t0 = time.perf_counter()
+    ref.set_parameter('Reynolds Number', mu)
+    F = ref.rhs(x)
+    ref.set_parameter('Reynolds Number', mu + 1.0)
+    dF = ref.rhs(x) - F
+    ref.set_parameter('Reynolds Number', mu)
+    J = ref.jacobian(x)
+    ref.solve(J, -F)
+    ref.solve(J, dF)
+    t_iter = time.perf_counter() - t0
+    est = t_iter * iters_total
+    return {
+        'value': accepted / est, 'unit': 'steps/s', 'cores': 1, 'kind': 'port',
+        'sample': ('one Newton-corrector iteration (2 rhs, 1 Jacobian, sparse LU via scipy splu, 2 solves) '
+                   'of the oracle at %dx%d, Re=%.0f, took %.2f s on %s (os.cpu_count()=%s, SuperLU single-threaded); '
+                   'scaled by the %d corrector/Newton iterations the same algorithm needed for %d accepted steps'
+                   % (n, n, mu, t_iter, platform.processor() or platform.machine(), os.cpu_count(),
+                      iters_total, accepted)),
+        'seconds_per_iteration': t_iter,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=0,
+                    help='accepted continuation steps to time (0: the full continuation to the target)')
+    ap.add_argument('--warmup', type=int, default=1, help='untimed warm-up continuation steps')
+    ap.add_argument('--n', type=int, default=256)
+    ap.add_argument('--target', type=float, default=2000.0)
+    ap.add_argument('--ds', type=float, default=100.0)
+    ap.add_argument('--variants', default='fp64,mixed')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build runs the continuation hot path on a single MI355X '
+              '(multi-GPU sharding is described in DESIGN.md, not implemented); use --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy
+    import torch
+    import transiflow_amd as tf
+
+    maxit = args.steps if args.steps > 0 else 1000
+    if args.warmup > 0:
+        log('warm-up: %d step(s) at %dx%d' % (args.warmup, args.n, args.n))
+        run_continuation(tf, torch, args.n, 'fp64', args.target, args.ds, args.warmup)
+
+    results = {}
+    for var in args.variants.split(','):
+        log('timed run: %s' % var)
+        results[var] = run_continuation(tf, torch, args.n, var, args.target, args.ds, maxit)
+        r = results[var]
+        log('  %s: %.2f s, %d accepted steps, %.3f steps/s, %d factorizations, %d GMRES its'
+            % (var, r['seconds'], r['accepted'], r['steps_per_s'], r['factorizations'], r['gmres_iterations']))
+
+    base = results.get('fp64')
+    xref = base['x'].numpy() if base else None
+    agree = {}
+    for var, r in results.items():
+        if base is not None and var != 'fp64':
+            xv = r['x'].numpy()
+            agree[var] = float(numpy.linalg.norm(xv - xref) / numpy.linalg.norm(xref))
+    ok = [v for v in results if v == 'fp64' or agree.get(v, 1.0) <= 1e-8]
+    head = max(ok, key=lambda v: results[v]['steps_per_s'])
+    hr = results[head]
+
+    log('kernel timing sample')
+    asm = assembly_roofline(tf, torch, hr['itf'], hr['x'])
+    kt, line_stats = solver_kernels(tf, torch, base['itf'] if base else hr['itf'], hr['x'])
+    # dominant kernel of the solve path
+    dom_name, (dom_calls, dom_ms) = max(kt.items(), key=lambda kv: kv[1][1])
+    m = line_stats['m']
+    if dom_name == 'gj_panel':
+        flops = 2.0 * 64 ** 3
+        roof = {'kernel': 'k_gj_panel (64x64 Gauss-Jordan panel inverse)', 'bound': 'mfma',
+                'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
+    elif dom_name == 'gj_update':
+        flops = 2.0 * 2 * 64 * m * m
+        roof = {'kernel': 'k_gj_update (rank-64 Gauss-Jordan update, FP64)', 'bound': 'mfma',
+                'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
+    elif dom_name == 'gemm':
+        flops = 2.0 * m ** 3
+        roof = {'kernel': 'dense FP64 block GEMM', 'bound': 'mfma', 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
+    else:
+        flops = None
+        roof = {'kernel': dom_name, 'bound': 'hbm', 'peak': HBM_PEAK_GBS, 'unit': 'GB/s'}
+    avg_s = dom_ms / max(dom_calls, 1) * 1e-3
+    if flops is not None:
+        achieved = flops / avg_s / 1e12
+    else:
+        achieved = 8.0 * m * m / avg_s / 1e9
+    roof.update({'achieved': achieved, 'frac': achieved / roof['peak'], 'traffic': None,
+                 'avg_launch_us': avg_s * 1e6, 'launches': dom_calls,
+                 'per_launch_work': flops if flops is not None else 8.0 * m * m})
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log('cpu baseline (oracle, bounded sample)')
+        iters_total = hr['corrector_iterations'] + hr['initial_newton_iterations'] + 1
+        cpu = cpu_baseline(args.n, hr['x'].numpy(), float(hr['mu']), iters_total, hr['accepted'])
+
+    out = {
+        'metric': METRIC,
+        'value': hr['steps_per_s'],
+        'unit': 'steps/s',
+        'n_gpus': 1,
+        'steps': hr['accepted'],
+        'warmup': args.warmup,
+        'ms_per_step': 1000.0 / hr['steps_per_s'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64' if head == 'fp64' else 'f64 state / f32-stored preconditioner',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength '
+                        'continuation Re 0->%g, ds=%g, Newton tol 1e-8' % (args.n, args.n, 3 * args.n ** 2,
+                                                                         args.target, args.ds),
+            'variant': head,
+            'parallelism': 'single GPU',
+        },
+        'variants': {v: {k: r[k] for k in ('steps_per_s', 'seconds', 'accepted', 'corrector_iterations',
+                                           'factorizations', 'solves', 'gmres_iterations',
+                                           'worst_relative_residual')} for v, r in results.items()},
+        'final_state_rel_err_vs_fp64': agree,
+        'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},
+        'assembly': asm,
+        'roofline': roof,
+        'solver_kernels_ms': {k: {'launches': c, 'total_ms': t} for k, (c, t) in kt.items()},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

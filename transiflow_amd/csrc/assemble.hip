@@ -230,6 +230,8 @@ int launch_assemble(tf_problem* p, const double* state, double* F, int* row_ptr,
   hipLaunchKernelGGL((k_assemble<DIM_, DOF_, BUOY_, MODE>), grid, block, 0, st, g, p->ph, state, F, p->counts_d, \
                      p->block_sums_d, p->block_off_d, row_ptr, col, val, p->ncell)
   const int key = p->dim * 100 + p->dof * 10 + p->buoy;
+  const int pid = MODE == MODE_RHS ? PROF_RHS : MODE == MODE_COUNT ? PROF_JAC_COUNT : PROF_JAC_FILL;
+  TF_PROF_BEGIN(pid, st);
   switch (key) {
     case 231: TF_ASM(2, 3, 1); break;
     case 241: TF_ASM(2, 4, 1); break;
@@ -241,6 +243,7 @@ int launch_assemble(tf_problem* p, const double* state, double* F, int* row_ptr,
     default: return fail("unsupported (dim, dof) combination for the HIP assembly");
   }
 #undef TF_ASM
+  TF_PROF_END(pid, st);
   TF_LAUNCH_CHECK("k_assemble");
   return 0;
 }

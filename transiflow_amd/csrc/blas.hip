@@ -5,7 +5,57 @@
 // reduces the partials; the same inputs give the same bits on every run.
 #include "tf_common.h"
 
+#include <mutex>
 #include <type_traits>
+#include <vector>
+
+namespace tf {
+
+namespace {
+struct ProfState {
+  bool on = false;
+  std::mutex mu;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  hipEvent_t open[PROF_IDS] = {};
+  long count[PROF_IDS] = {};
+  double ms[PROF_IDS] = {};
+};
+ProfState& ps() {
+  static ProfState s;
+  return s;
+}
+hipEvent_t take_event(ProfState& s) {
+  if (!s.pool.empty()) {
+    hipEvent_t e = s.pool.back();
+    s.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+}  // namespace
+
+bool prof_on() { return ps().on; }
+
+void prof_begin(int id, hipStream_t st) {
+  ProfState& s = ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  hipEvent_t e = take_event(s);
+  (void)hipEventRecord(e, st);
+  s.open[id] = e;
+}
+
+void prof_end(int id, hipStream_t st) {
+  ProfState& s = ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  hipEvent_t e = take_event(s);
+  (void)hipEventRecord(e, st);
+  s.pending.push_back({id, {s.open[id], e}});
+}
+
+}  // namespace tf
 
 namespace {
 
@@ -107,6 +157,44 @@ int grid_for(long n) {
 
 extern "C" {
 
+int tf_prof_enable(int on) {
+  tf::ps().on = on != 0;
+  return 0;
+}
+
+// Synchronises the recorded events; count/ms_total receive the totals of kernel id.
+int tf_prof_read(int id, long* count, double* ms_total) {
+  if (id < 0 || id >= tf::PROF_IDS) return tf::fail("bad profiling id");
+  tf::ProfState& s = tf::ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  for (auto& p : s.pending) {
+    (void)hipEventSynchronize(p.second.second);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    s.count[p.first] += 1;
+    s.ms[p.first] += ms;
+    s.pool.push_back(p.second.first);
+    s.pool.push_back(p.second.second);
+  }
+  s.pending.clear();
+  *count = s.count[id];
+  *ms_total = s.ms[id];
+  return 0;
+}
+
+int tf_prof_reset(void) {
+  long c;
+  double m;
+  tf_prof_read(0, &c, &m);
+  tf::ProfState& s = tf::ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  for (int i = 0; i < tf::PROF_IDS; ++i) {
+    s.count[i] = 0;
+    s.ms[i] = 0.0;
+  }
+  return 0;
+}
+
 int tf_set_entry_f64(double* x, long i, double v, void* stream) {
   hipLaunchKernelGGL(k_set_entry, dim3(1), dim3(1), 0, (hipStream_t)stream, x, i, v);
   TF_LAUNCH_CHECK("k_set_entry");
@@ -169,8 +257,10 @@ int tf_dot_f64(long n, const double* x, const double* y, double* work, double* o
 
 int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const double* x, double* y, int gauge,
                 void* stream) {
+  TF_PROF_BEGIN(tf::PROF_SPMV, (hipStream_t)stream);
   hipLaunchKernelGGL((k_spmv<double, double>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
                      x, y, gauge);
+  TF_PROF_END(tf::PROF_SPMV, (hipStream_t)stream);
   TF_LAUNCH_CHECK("k_spmv f64");
   return 0;
 }

@@ -141,151 +141,189 @@ __global__ __launch_bounds__(256) void k_schur(int m, int j, const Coupling* __r
 
 // ---- blocked Gauss-Jordan inversion of X (m x m, FP64, in place) -------------
 
-// Invert the pb x pb diagonal panel block: Gauss-Jordan on [A | I] with partial
-// pivoting among the block's rows (the swaps act on both halves, so the right
-// half ends as A^{-1} itself and no permutation leaks out of the panel).
+// Invert the pb x pb diagonal panel block (pb <= 64; rows/columns beyond pb are
+// padded with the identity).  Gauss-Jordan on [A | I] with partial pivoting,
+// implicit (no row swaps): 8 waves, lane = row, each wave holds 16 of the 128
+// augmented columns in registers.  Per column: the owner wave publishes the
+// column through LDS, every wave finds the same pivot row with shuffles, reads
+// the pivot row with readlane and eliminates its own columns.  The row that
+// pivoted column k ends holding row k of the inverse in the right half.
 // Pivots below tol * max|block| are replaced by +-tol*max (counted).
-__global__ __launch_bounds__(256) void k_gj_panel(int m, int k0, int pb, const double* __restrict__ X,
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(512) void k_gj_panel(int m, int k0, int pb, const double* __restrict__ X,
                                                   double* __restrict__ Pinv, int* __restrict__ nperturb) {
-  __shared__ double A[PB][2 * PB + 1];
-  __shared__ double fac[PB];
-  __shared__ double red[256];
-  __shared__ int redi[256];
-  const int t = threadIdx.x;
-  for (int e = t; e < pb * pb; e += blockDim.x) {
-    const int r = e / pb, c = e % pb;
-    A[r][c] = X[(long)(k0 + r) * m + k0 + c];
-    A[r][pb + c] = r == c ? 1.0 : 0.0;
+  __shared__ double colk[2][64];
+  __shared__ double wmax[8];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = wave * 16 + q;
+    double v;
+    if (c < 64)
+      v = (lane < pb && c < pb) ? X[(long)(k0 + lane) * m + k0 + c] : (lane == c ? 1.0 : 0.0);
+    else
+      v = (lane == c - 64) ? 1.0 : 0.0;
+    a[q] = v;
   }
-  __syncthreads();
-  // scale for the perturbation threshold
   double mx = 0.0;
-  for (int e = t; e < pb * pb; e += blockDim.x) mx = fmax(mx, fabs(A[e / pb][e % pb]));
-  red[t] = mx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) red[t] = fmax(red[t], red[t + s]);
-    __syncthreads();
+  if (wave < 4) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(a[q]));
   }
-  const double amax = red[0] > 0.0 ? red[0] : 1.0;
-  const double tol = 1e-13 * amax;
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) wmax[wave] = mx;
   __syncthreads();
-  for (int k = 0; k < pb; ++k) {
-    // pivot search over rows k..pb-1
-    double best = -1.0;
-    int bi = k;
-    if (t >= k && t < pb) {
-      best = fabs(A[t][k]);
-      bi = t;
+  double amax = 0.0;
+  for (int w = 0; w < 4; ++w) amax = fmax(amax, wmax[w]);
+  if (amax == 0.0) amax = 1.0;
+  const double tol = 1e-13 * amax;
+  bool used = false;
+  int mystep = -1;
+  for (int k = 0; k < 64; ++k) {
+    const int owner = k >> 4;
+    if (wave == owner) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q == (k & 15)) colk[k & 1][lane] = a[q];
     }
-    red[t] = best;
-    redi[t] = bi;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (t < s && red[t + s] > red[t]) {
-        red[t] = red[t + s];
-        redi[t] = redi[t + s];
-      }
-      __syncthreads();
-    }
-    const int p = redi[0];
-    __syncthreads();
-    if (p != k) {
-      for (int c = t; c < 2 * pb; c += blockDim.x) {
-        const double tmp = A[k][c];
-        A[k][c] = A[p][c];
-        A[p][c] = tmp;
+    const double f = colk[k & 1][lane];
+    double best = used ? -1.0 : fabs(f);
+    int bi = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
       }
     }
-    __syncthreads();
-    double piv = A[k][k];
+    const int p = __builtin_amdgcn_readfirstlane(bi);
+    double piv = readlane_f64(f, p);
     if (fabs(piv) < tol) {
       piv = piv < 0.0 ? -tol : tol;
-      if (t == 0) atomicAdd(nperturb, 1);
+      if (threadIdx.x == 0) atomicAdd(nperturb, 1);
     }
     const double ip = 1.0 / piv;
-    __syncthreads();
-    for (int c = t; c < 2 * pb; c += blockDim.x) A[k][c] = (c == k) ? 1.0 : A[k][c] * ip;
-    __syncthreads();
-    if (t < pb) fac[t] = A[t][k];
-    __syncthreads();
-    for (int e = t; e < pb * 2 * pb; e += blockDim.x) {
-      const int r = e / (2 * pb), c = e % (2 * pb);
-      if (r != k) A[r][c] = (c == k) ? 0.0 : A[r][c] - fac[r] * A[k][c];
+    if (lane == p) {
+      used = true;
+      mystep = k;
     }
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const double pr = readlane_f64(a[q], p) * ip;
+      a[q] = (lane == p) ? pr : a[q] - f * pr;
+    }
   }
-  for (int e = t; e < pb * pb; e += blockDim.x) Pinv[e] = A[e / pb][pb + e % pb];
-}
-
-// T[t][c] = sum_s Pinv[t][s] X[k0+s][c] for c outside the panel; Uc[i][s] = X[i][k0+s]
-__global__ __launch_bounds__(256) void k_gj_rows(int m, int k0, int pb, const double* __restrict__ X,
-                                                 const double* __restrict__ Pinv, double* __restrict__ Tm,
-                                                 double* __restrict__ Uc) {
-  __shared__ double P[PB * PB];
-  for (int e = threadIdx.x; e < pb * pb; e += blockDim.x) P[e] = Pinv[e];
-  __syncthreads();
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < m) {
-    if (c < k0 || c >= k0 + pb) {
-      double col[PB];
-      for (int s = 0; s < pb; ++s) col[s] = X[(long)(k0 + s) * m + c];
-      for (int tt = 0; tt < pb; ++tt) {
-        double acc = 0.0;
-        for (int s = 0; s < pb; ++s) acc += P[tt * pb + s] * col[s];
-        Tm[(long)tt * m + c] = acc;
-      }
+  if (wave >= 4 && mystep >= 0 && mystep < pb) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = wave * 16 + q - 64;
+      if (c < pb) Pinv[mystep * pb + c] = a[q];
     }
-    // c indexes rows here for the Uc copy
-    if (c < k0 || c >= k0 + pb)
-      for (int s = 0; s < pb; ++s) Uc[(long)c * PB + s] = X[(long)c * m + k0 + s];
   }
 }
 
-// Rank-pb update of all of X (64x64 output tiles; the panel rows/columns get
-// their Gauss-Jordan values, the rest X_OR -= Uc T).
+// Snapshot of the panel rows (Kr = X[k0:k0+pb, :]) and columns (Kc = X[:, k0:k0+pb])
+// before the in-place update overwrites them.
+__global__ __launch_bounds__(256) void k_gj_copy(int m, int k0, int pb, const double* __restrict__ X,
+                                                 double* __restrict__ Kr, double* __restrict__ Kc) {
+  const long total = (long)pb * m;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int s = (int)(e / m), c = (int)(e % m);
+    Kr[(long)s * m + c] = X[(long)(k0 + s) * m + c];
+    const int i = (int)(e / pb), t = (int)(e % pb);
+    Kc[(long)i * PB + t] = X[(long)i * m + k0 + t];
+  }
+}
+
+// One 64x64 output tile of the block Gauss-Jordan step for panel K:
+//   K rows:     X[K, K] = Pinv,  X[K, R] = T = Pinv X[K, R]
+//   other rows: X[O, K] = -X[O, K] Pinv,  X[O, R] -= X[O, K] T
+// T's tile is recomputed by every row tile from the snapshots (cheap, no extra pass).
 __global__ __launch_bounds__(256) void k_gj_update(int m, int k0, int pb, double* __restrict__ X,
-                                                   const double* __restrict__ Pinv, const double* __restrict__ Tm,
-                                                   const double* __restrict__ Uc) {
-  __shared__ double Us[64][PB + 1];
-  __shared__ double Ts[PB][64 + 1];
+                                                   const double* __restrict__ Pinv, const double* __restrict__ Kr,
+                                                   const double* __restrict__ Kc) {
+  __shared__ double P[PB][PB + 1];
+  __shared__ double B[PB][64 + 1];   // Kr tile, then T tile (or Pinv columns)
+  __shared__ double U[64][PB + 1];   // Kc tile
   const int i0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
   const int t = threadIdx.x;
-  const bool rowsK = (i0 >= k0 && i0 < k0 + pb);  // tiles are 64-aligned like the panels
-  const bool colsK = (c0 >= k0 && c0 < k0 + pb);
-  // stage the left factor: Uc rows (or Pinv for K rows) and the right factor
-  for (int e = t; e < 64 * pb; e += 256) {
-    const int a = e / pb, s = e % pb;
-    const int i = i0 + a;
-    Us[a][s] = (i < m && !rowsK) ? Uc[(long)i * PB + s] : 0.0;
+  const bool rowsK = (i0 == k0);
+  const bool colsK = (c0 == k0);
+  const int tr = (t >> 4) * 4, tc = (t & 15) * 4;
+  for (int e = t; e < PB * PB; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    P[r][c] = (r < pb && c < pb) ? Pinv[r * pb + c] : 0.0;
   }
-  for (int e = t; e < pb * 64; e += 256) {
-    const int s = e / 64, b = e % 64;
-    const int c = c0 + b;
-    double val = 0.0;
-    if (c < m) val = colsK ? Pinv[s * pb + (c - k0)] : Tm[(long)s * m + c];
-    Ts[s][b] = val;
-  }
-  __syncthreads();
-  // each thread: 4 x 4 outputs
-  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  if (!rowsK) {
-    for (int s = 0; s < pb; ++s) {
+  if (colsK) {
+    for (int e = t; e < PB * 64; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      B[r][c] = (r < pb && c < pb) ? Pinv[r * pb + c] : 0.0;
+    }
+    __syncthreads();
+  } else {
+    for (int e = t; e < PB * 64; e += 256) {
+      const int s = e >> 6, b = e & 63;
+      B[s][b] = (s < pb && c0 + b < m) ? Kr[(long)s * m + c0 + b] : 0.0;
+    }
+    __syncthreads();
+    // T tile = P * B, 4x4 per thread, then written back into B
+    double acc[4][4] = {};
+    for (int s = 0; s < PB; ++s) {
       double u[4], w[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) u[a] = Us[tr + a][s];
+      for (int a = 0; a < 4; ++a) u[a] = P[tr + a][s];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) w[b] = Ts[s][tc + b];
+      for (int b = 0; b < 4; ++b) w[b] = B[s][tc + b];
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * w[b];
     }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) B[tr + a][tc + b] = acc[a][b];
+    __syncthreads();
+  }
+  if (rowsK) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = i0 + tr + a;
+      if (tr + a >= pb || i >= m) continue;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int c = c0 + tc + b;
+        if (c < m) X[(long)i * m + c] = B[tr + a][tc + b];
+      }
+    }
+    return;
+  }
+  for (int e = t; e < 64 * PB; e += 256) {
+    const int a = e >> 6, s = e & 63;
+    U[a][s] = (i0 + a < m && s < pb) ? Kc[(long)(i0 + a) * PB + s] : 0.0;
+  }
+  __syncthreads();
+  double acc[4][4] = {};
+  for (int s = 0; s < PB; ++s) {
+    double u[4], w[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) u[a] = U[tr + a][s];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) w[b] = B[s][tc + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += u[a] * w[b];
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -296,12 +334,7 @@ __global__ __launch_bounds__(256) void k_gj_update(int m, int k0, int pb, double
       const int c = c0 + tc + b;
       if (c >= m) continue;
       double* x = X + (long)i * m + c;
-      if (rowsK)
-        *x = Ts[i - k0][tc + b];  // panel rows: Pinv (K cols) or T (other cols)
-      else if (colsK)
-        *x = -acc[a][b];          // X_OK = -Uc Pinv
-      else
-        *x = *x - acc[a][b];      // X_OR -= Uc T
+      *x = colsK ? -acc[a][b] : *x - acc[a][b];
     }
   }
 }
@@ -418,9 +451,13 @@ int factor_impl(tf_linelu* h, const int* rp, const int* ci, const double* v, int
     if (j > 0) hipLaunchKernelGGL((k_schur<T>), dim3(m), dim3(256), 0, st, m, j, h->Lc, h->lcount, W + (j - 1) * mm, h->X);
     for (int k0 = 0; k0 < m; k0 += PB) {
       const int pb = m - k0 < PB ? m - k0 : PB;
-      hipLaunchKernelGGL(k_gj_panel, dim3(1), dim3(256), 0, st, m, k0, pb, h->X, h->Pinv, h->nperturb);
-      hipLaunchKernelGGL(k_gj_rows, dim3((m + 255) / 256), dim3(256), 0, st, m, k0, pb, h->X, h->Pinv, h->Tm, h->Uc);
+      TF_PROF_BEGIN(tf::PROF_GJ_PANEL, st);
+      hipLaunchKernelGGL(k_gj_panel, dim3(1), dim3(512), 0, st, m, k0, pb, h->X, h->Pinv, h->nperturb);
+      TF_PROF_END(tf::PROF_GJ_PANEL, st);
+      hipLaunchKernelGGL(k_gj_copy, dim3(((long)pb * m + 255) / 256), dim3(256), 0, st, m, k0, pb, h->X, h->Tm, h->Uc);
+      TF_PROF_BEGIN(tf::PROF_GJ_UPDATE, st);
       hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, st, m, k0, pb, h->X, h->Pinv, h->Tm, h->Uc);
+      TF_PROF_END(tf::PROF_GJ_UPDATE, st);
     }
     hipLaunchKernelGGL((k_store_inverse<T>), dim3(m), dim3(256), 0, st, m, h->X, Y + j * mm);
     if (j + 1 < N) hipLaunchKernelGGL((k_make_W<T>), dim3(m), dim3(256), 0, st, m, j, Y + j * mm, h->Ut, h->ucount, W + j * mm);

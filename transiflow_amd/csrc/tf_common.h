@@ -27,7 +27,36 @@ inline int fail_hip(hipError_t e, const char* where) {
   return -2;
 }
 
+// ---- opt-in kernel timing -------------------------------------------------
+// When enabled (tf_prof_enable), launches wrapped in TF_PROF_BEGIN/END record
+// a pair of HIP events on the launch stream; tf_prof_read() synchronises and
+// returns the launch count and summed duration per kernel id.
+enum ProfId : int {
+  PROF_JAC_COUNT = 0,
+  PROF_JAC_FILL = 1,
+  PROF_RHS = 2,
+  PROF_GJ_PANEL = 3,
+  PROF_GJ_UPDATE = 4,
+  PROF_GEMM = 5,
+  PROF_GEMV = 6,
+  PROF_SPMV = 7,
+  PROF_CONV = 8,
+  PROF_IDS = 16
+};
+bool prof_on();
+void prof_begin(int id, hipStream_t st);
+void prof_end(int id, hipStream_t st);
+
 }  // namespace tf
+
+#define TF_PROF_BEGIN(id, st) \
+  do {                        \
+    if (tf::prof_on()) tf::prof_begin((id), (st)); \
+  } while (0)
+#define TF_PROF_END(id, st) \
+  do {                      \
+    if (tf::prof_on()) tf::prof_end((id), (st)); \
+  } while (0)
 
 #define TF_HIP(call)                                      \
   do {                                                    \

@@ -294,5 +294,10 @@ class Solver:
                               % (rel, its))
         return DeviceVector(x)
 
+    def line_stats(self):
+        """Shape of the line factorisation for this problem (rows per line, lines)."""
+        m = line_block_size(self.itf)
+        return {'m': m, 'lines': self.itf.n // m}
+
     def solve_bordered(self, jac, rhs, rhs2, V, W, C):
         raise NotImplementedError('bordered solves are not available yet in the HIP backend')
