@@ -95,6 +95,46 @@ int tf_spmv_f64(int n, const int* row_ptr, const int* col, const double* val, co
                 double* y, int gauge, void* stream);
 int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, const float* x,
                 float* y, int gauge, void* stream);
+int tf_set_entry_f64(double* x, long i, double v, void* stream);
+int tf_fill_f64(long n, double v, double* x, void* stream);
+
+/* ---- Krylov building blocks (scipy.sparse.linalg.gmres in SciPy.py:343-370) ----
+ * V is a row-major basis with leading dimension ldv; h/coef are device FP64. */
+int tf_krylov_work_size(void);
+int tf_krylov_max_basis(void);
+int tf_multidot_f64(int n, int k, const double* V, long ldv, const double* w, double* work, double* h,
+                    int accumulate, void* stream);
+int tf_multidot_f32(int n, int k, const float* V, long ldv, const float* w, double* work, double* h,
+                    int accumulate, void* stream);
+int tf_multiaxpy_f64(int n, int k, const double* V, long ldv, const double* coef, double* w, double sign,
+                     void* stream);
+int tf_multiaxpy_f32(int n, int k, const float* V, long ldv, const double* coef, float* w, double sign,
+                     void* stream);
+int tf_normalize_f64(int n, const double* w, double* work, double* nrm_dev, double* out, void* stream);
+int tf_normalize_f32(int n, const float* w, double* work, double* nrm_dev, float* out, void* stream);
+
+/* Cell-block Jacobi preconditioner (dof x dof diagonal blocks of the gauged matrix). */
+int tf_cellblock_setup(int ncell, int dof, const int* row_ptr, const int* col, const double* val,
+                       int gauge, double* inv, void* stream);
+int tf_cellblock_apply_f64(int ncell, int dof, const double* inv, const double* r, double* z, void* stream);
+int tf_cellblock_apply_f32(int ncell, int dof, const double* inv, const float* r, float* z, void* stream);
+
+/* ---- exact block-tridiagonal factorisation over grid lines ----
+ * Replaces the sparse LU of SciPy._compute_factorization / _lu_solve (SciPy.py:131-162,194-202)
+ * as the preconditioner of GMRES.  n = lines * m; single != 0 stores the factor in FP32. */
+typedef struct tf_linelu tf_linelu;
+int tf_linelu_create(tf_linelu** out, int n, int m, int single);
+int tf_linelu_destroy(tf_linelu* h);
+int tf_linelu_factor(tf_linelu* h, const int* row_ptr, const int* col, const double* val, int gauge,
+                     void* stream);
+int tf_linelu_status(tf_linelu* h, int* flags_host, void* stream);
+int tf_linelu_solve_f64(tf_linelu* h, const double* r, double* x, void* stream);
+int tf_linelu_solve_f32(tf_linelu* h, const float* r, float* x, void* stream);
+
+/* ---- opt-in kernel timing with HIP events (no reference counterpart; used by bench.py) ---- */
+int tf_prof_enable(int on);
+int tf_prof_read(int id, long* count, double* ms_total);
+int tf_prof_reset(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,221 @@
+"""CPU checks of the oracle against the reference's own fixtures and golden vectors.
+
+The oracle (oracle/) restates the reference algorithm; these tests pin it:
+reference known-answer matrices (exact where the reference compares with
+``==``: tests/test_fvm.py:547-841), full J/F fixtures (approx, as the
+reference), the golden vectors written from the reference for the 2D paths
+(tests/golden/make_golden.py), the 2D == 3D(nz=1) equivalence
+(tests/test_fvm.py:919-968), finite-difference Jacobian consistency
+(tests/test_jacobian.py:139-230), the divergence-free zero-diagonal invariant
+(tests/test_jacobian.py:93-137) and the coordinate-vector layout
+(tests/test_fvm.py:31-59).
+"""
+import os
+
+import numpy
+import pytest
+
+from oracle import layout
+from oracle.crs import assemble_from_atom
+from oracle.discretization import Discretization
+from tests.golden.fixtures import read_matrix, read_vector
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _ramp(n):
+    return numpy.arange(1, n + 1, dtype=float)
+
+
+def _assert_matrix(A, name, exact, skip_bc_rows=False):
+    begB, jB, cB = read_matrix(name)
+    n = len(begB) - 1
+    for i in range(n):
+        a = slice(A.begA[i], A.begA[i + 1])
+        b = slice(begB[i], begB[i + 1])
+        if skip_bc_rows and A.begA[i + 1] - A.begA[i] == 1 and A.coA[A.begA[i]] == -1:
+            continue
+        assert numpy.array_equal(A.jcoA[a], jB[b]), 'row %d pattern' % i
+        if exact:
+            assert numpy.array_equal(A.coA[a], cB[b]), 'row %d values' % i
+        else:
+            numpy.testing.assert_allclose(A.coA[a], cB[b], rtol=1e-6, atol=1e-12)
+
+
+LDC = {'Reynolds Number': 100}
+BOUS = {'Reynolds Number': 1, 'Rayleigh Number': 100, 'Prandtl Number': 100}
+
+
+def test_ldc_lin_exact():
+    d = Discretization(dict(LDC), 4, 4, 4, 3, 4)
+    _assert_matrix(assemble_from_atom(d.linear_part(), wrap=False), 'ldc_lin_4x4x4.txt', exact=True)
+
+
+def test_bous_lin_exact():
+    d = Discretization(dict(BOUS), 4, 4, 4, 3, 5)
+    _assert_matrix(assemble_from_atom(d.linear_part(), wrap=False), 'bous_lin_4x4x4.txt', exact=True)
+
+
+def test_ldc_bnd_exact():
+    d = Discretization(dict(LDC), 4, 4, 4, 3, 4)
+    a = d.linear_part().copy()
+    d.boundaries(a)
+    _assert_matrix(assemble_from_atom(a), 'ldc_bnd_4x4x4.txt', exact=True)
+
+
+def test_bous_bnd():
+    d = Discretization(dict(BOUS, **{'Problem Type': 'Rayleigh-Benard'}), 4, 4, 4, 3, 5)
+    a = d.linear_part().copy()
+    d.boundaries(a)
+    _assert_matrix(assemble_from_atom(a), 'bous_bnd_4x4x4.txt', exact=False)
+
+
+@pytest.mark.parametrize('name,params,dof', [('ldc_bil', LDC, 4), ('bous_bil', BOUS, 5)])
+def test_bil_exact(name, params, dof):
+    d = Discretization(dict(params), 4, 4, 4, 3, dof)
+    aj, _ = d.nonlinear_part(_ramp(64 * dof))
+    d.boundaries(aj)
+    _assert_matrix(assemble_from_atom(aj), name + '_4x4x4.txt', exact=True, skip_bc_rows=True)
+
+
+@pytest.mark.parametrize('name,params,dof', [
+    ('ldc', LDC, 4),
+    ('ldc_stretched', dict(LDC, **{'Grid Stretching': True}), 4),
+    ('bous', dict(BOUS, **{'Problem Type': 'Rayleigh-Benard'}), 5),
+    ('bous_stretched', dict(BOUS, **{'Problem Type': 'Rayleigh-Benard', 'Grid Stretching': True}), 5),
+    ('dhc', dict(BOUS, **{'Problem Type': 'Differentially Heated Cavity'}), 5),
+])
+def test_full_fixtures(name, params, dof):
+    d = Discretization(dict(params), 4, 4, 4, 3, dof)
+    state = _ramp(64 * dof)
+    _assert_matrix(d.jacobian(state), name + '_4x4x4.txt', exact=False)
+    numpy.testing.assert_allclose(d.rhs(state), read_vector(name + '_rhs_4x4x4.txt'), rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize('case', ['ldc2d_16', 'rb2d_16x8', 'ldc3d_semi'])
+def test_golden_vectors_exact(case):
+    g = numpy.load(os.path.join(GOLDEN, case + '.npz'))
+    if case == 'ldc2d_16':
+        d = Discretization({'Reynolds Number': 100}, 16, 16, 1, 2, 3)
+    elif case == 'rb2d_16x8':
+        d = Discretization({'Problem Type': 'Rayleigh-Benard', 'Prandtl Number': 1.0, 'Rayleigh Number': 1000.0,
+                            'Biot Number': 1.0, 'X-max': 4.0, 'Reynolds Number': 1.0}, 16, 8, 1, 2, 4)
+    else:
+        d = Discretization({'Reynolds Number': 50}, 6, 5, 1, 3, 4)
+    assert numpy.array_equal(d.rhs(g['state']), g['F'])
+    J = d.jacobian(g['state'])
+    assert numpy.array_equal(J.begA, g['begA'])
+    assert numpy.array_equal(J.jcoA, g['jcoA'])
+    assert numpy.array_equal(J.coA, g['coA'])
+
+
+@pytest.mark.parametrize('ptype,dof3,dof2,shift', [('Lid-driven Cavity', 4, 3, 1), ('Rayleigh-Benard', 5, 4, 2)])
+def test_2d_equals_3d_nz1(ptype, dof3, dof2, shift):
+    """tests/test_fvm.py:919-968,1048-1098: the 2D formulation is the 3D one with
+    nz = 1 after dropping w (index remapping i -> i + (i + shift) // dof2)."""
+    params = {'Reynolds Number': 100, 'Problem Type': ptype, 'Rayleigh Number': 100, 'Prandtl Number': 100}
+    n1 = 16 * dof3
+    s1 = _ramp(n1)
+    d1 = Discretization(dict(params), 4, 4, 1, 3, dof3)
+    A1 = d1.jacobian(s1)
+    r1 = d1.rhs(s1)
+    n2 = 16 * dof2
+    s2 = numpy.array([s1[i + (i + shift) // dof2] for i in range(n2)])
+    d2 = Discretization(dict(params), 4, 4, 1, 2, dof2)
+    A2 = d2.jacobian(s2)
+    r2 = d2.rhs(s2)
+    for i in range(n2):
+        i1 = i + (i + shift) // dof2
+        assert A1.begA[i1 + 1] - A1.begA[i1] == A2.begA[i + 1] - A2.begA[i]
+        for j in range(A2.begA[i + 1] - A2.begA[i]):
+            j1 = A1.begA[i1] + j
+            j2 = A2.begA[i] + j
+            assert A1.jcoA[j1] == A2.jcoA[j2] + (A2.jcoA[j2] + shift) // dof2
+            assert A1.coA[j1] == pytest.approx(A2.coA[j2])
+        assert r2[i] == pytest.approx(r1[i1])
+
+
+def _random_state(nx, ny, nz, dim, dof, rng):
+    x = layout.state_mtx(rng.random(nx * ny * nz * dof), nx, ny, nz, dof)
+    x[nx - 1, :, :, 0] = 0
+    x[:, ny - 1, :, 1] = 0
+    if dim > 2:
+        x[:, :, nz - 1, 2] = 0
+    return layout.state_vec(x)
+
+
+@pytest.mark.parametrize('shape,dim,dof,stretch', [
+    ((13, 7, 5), 3, 5, False), ((13, 7, 1), 2, 3, False), ((13, 7, 1), 2, 3, True), ((9, 6, 4), 3, 4, True),
+])
+def test_fd_jacobian_consistency(shape, dim, dof, stretch):
+    """tests/test_jacobian.py:139-230."""
+    rng = numpy.random.default_rng(7)
+    params = {'Reynolds Number': 100}
+    if stretch:
+        params['Grid Stretching'] = True
+    d = Discretization(params, *shape, dim, dof)
+    state = _random_state(*shape, dim, dof, rng)
+    pert = _random_state(*shape, dim, dof, rng)
+    A = d.jacobian(state).to_scipy()
+    rhs = d.rhs(state)
+    for i in range(3, 12):
+        eps = 10.0 ** -i
+        eps2 = max(eps, 10.0 ** (-14 + i))
+        assert numpy.linalg.norm((d.rhs(state + eps * pert) - rhs) / eps - A @ pert) < eps2
+
+
+def test_divfree_zero_diagonal():
+    """tests/test_jacobian.py:93-137: the convective atom of a divergence-free
+    state has no diagonal except the -1 boundary rows."""
+    rng = numpy.random.default_rng(3)
+    nx, ny, nz, dim, dof = 6, 5, 4, 3, 4
+    d = Discretization({'Reynolds Number': 100}, nx, ny, nz, dim, dof)
+    state = _random_state(nx, ny, nz, dim, dof, rng)
+    A = d.jacobian(state)
+    p = numpy.zeros((A.n, A.n // dof))
+    for i in range(A.n):
+        if i % dof == dim:
+            for j in range(A.begA[i], A.begA[i + 1]):
+                p[A.jcoA[j], i // dof] = A.coA[j]
+    state = state - p @ numpy.linalg.solve(p.T @ p, p.T @ state)
+    _, atomF = d.nonlinear_part(state)
+    d.boundaries(atomF)
+    B = assemble_from_atom(atomF)
+    for i in range(B.n):
+        for j in range(B.begA[i], B.begA[i + 1]):
+            assert i != B.jcoA[j] or B.coA[j] == -1
+
+
+def test_coordinate_vectors():
+    """tests/test_fvm.py:31-59."""
+    x = layout.uniform_coordinate_vector(0, 1, 5)
+    for i in range(5):
+        assert x[i] == pytest.approx((i + 1) / 5)
+    assert x[-1] == pytest.approx(0) and x[0] > 0 and x[4] == pytest.approx(1)
+    x = layout.uniform_coordinate_vector(1, 4, 5)
+    for i in range(5):
+        assert x[i] == pytest.approx((i + 1) * 3 / 5 + 1)
+    x = layout.stretched_coordinate_vector(0, 1, 5, 1.5)
+    assert x[-1] == pytest.approx(0) and x[0] > 0 and x[4] == pytest.approx(1)
+    assert x[1] - x[0] < x[2] - x[1]
+
+
+def test_state_layout_roundtrip():
+    rng = numpy.random.default_rng(0)
+    v = rng.random(5 * 4 * 3 * 4)
+    m = layout.state_mtx(v, 5, 4, 3, 4)
+    assert m[2, 1, 0, 3] == v[3 + 2 * 4 + 1 * 4 * 5]
+    assert numpy.array_equal(layout.state_vec(m), v)
+
+
+def test_mass_matrix_closed_form():
+    """Discretization.py:417-437: diagonal cell volumes; pressure rows empty."""
+    d = Discretization({'Grid Stretching': True}, 5, 4, 3, 3, 5)
+    M = d.mass_matrix()
+    x, y, z = d.x, d.y, d.z
+    vol_u = (x[2] - x[0]) / 2 * (y[1] - y[0]) * (z[0] - z[-1])  # u of cell (1, 1, 0)
+    row = ((0 * 4 + 1) * 5 + 1) * 5
+    assert M.begA[row + 1] - M.begA[row] == 1
+    assert M.coA[M.begA[row]] == pytest.approx(vol_u)
+    prow = row + 3
+    assert M.begA[prow + 1] == M.begA[prow]
