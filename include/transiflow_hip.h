@@ -131,6 +131,16 @@ int tf_linelu_status(tf_linelu* h, int* flags_host, void* stream);
 int tf_linelu_solve_f64(tf_linelu* h, const double* r, double* x, void* stream);
 int tf_linelu_solve_f32(tf_linelu* h, const float* r, float* x, void* stream);
 
+/* ---- block cyclic reduction over grid lines (same role as tf_linelu, log-depth) ----
+ * Odd-even reduction of the block-tridiagonal gauged Jacobian; every level is a batch of
+ * dense inversions and GEMMs.  single != 0 factors and stores in FP32. */
+typedef struct tf_bcr tf_bcr;
+int tf_bcr_create(tf_bcr** out, int n, int m, int single);
+int tf_bcr_destroy(tf_bcr* h);
+int tf_bcr_factor(tf_bcr* h, const int* row_ptr, const int* col, const double* val, int gauge, void* stream);
+int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);
+int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
+
 /* ---- opt-in kernel timing with HIP events (no reference counterpart; used by bench.py) ---- */
 int tf_prof_enable(int on);
 int tf_prof_read(int id, long* count, double* ms_total);

@@ -1,7 +1,8 @@
 """GPU linear solves and Newton/continuation parity.
 
 solve(): restarted GMRES on the pressure-gauged Jacobian, preconditioned by
-the exact block-tridiagonal line factorisation, must reach a relative
+the exact block-tridiagonal line factorisation (block cyclic reduction, and
+the Thomas recursion for comparison), must reach a relative
 residual of 1e-10 and agree with the oracle's sparse-LU (scipy splu) solution
 of the same gauged system (SciPy.py:204-258 semantics) to 1e-8 relative.
 Newton and the pseudo-arclength continuation are compared with vectors
@@ -53,6 +54,21 @@ def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64
 ])
 def test_solve_line(hip, label, params, shape, dim, dof, seed):
     _solve_case(hip, params, shape, dim, dof, seed)
+
+
+@pytest.mark.parametrize('shape', [(24, 17, 1), (20, 5, 1), (12, 2, 1)])
+def test_solve_line_odd_counts(hip, shape):
+    """Reduction schedules with odd active counts at several levels (17 = 8+8+1 ...),
+    and a two-line grid."""
+    _solve_case(hip, {'Reynolds Number': 400}, shape, 2, 3, 11)
+
+
+@pytest.mark.parametrize('label,params,shape,dim,dof,seed', [
+    ('ldc2d_64x40', {'Reynolds Number': 2000}, (64, 40, 1), 2, 3, 2),
+    ('ldc3d_6', {'Reynolds Number': 100}, (6, 6, 6), 3, 4, 5),
+])
+def test_solve_thomas(hip, label, params, shape, dim, dof, seed):
+    _solve_case(hip, params, shape, dim, dof, seed, prec='thomas')
 
 
 def test_solve_full_size_ldc256(hip):

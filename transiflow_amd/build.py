@@ -39,6 +39,7 @@ SOURCES = {
     'blas.hip': [],
     'krylov.hip': [],
     'linelu.hip': [],
+    'bcr.hip': [],
 }
 
 

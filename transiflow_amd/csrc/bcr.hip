@@ -1,0 +1,946 @@
+// Block cyclic reduction (BCR) of the gauged Jacobian over grid lines.
+//
+// The Jacobian is block tridiagonal over grid lines (see linelu.hip): line j
+// couples to lines j-1 (L_j) and j+1 (U_j).  Odd-even reduction eliminates the
+// lines at even positions of the active list, level by level, until one line
+// is left.  For an eliminated line e with kept neighbours a (before) and b
+// (after):
+//
+//   x_e = Y_e (r_e - L_e x_a - U_e x_b),   Y_e = D_e^{-1}
+//
+// and a kept line k with eliminated neighbours e1 (before) and e2 (after) is
+// updated to
+//
+//   P1 = L_k Y_e1,  P2 = U_k Y_e2
+//   D_k <- D_k - P1 U_e1 - P2 L_e2,   L_k <- -P1 L_e1,   U_k <- -P2 U_e2
+//   r_k <- r_k - P1 r_e1 - P2 r_e2
+//
+// Every level is a batch of independent dense operations: the inversions of
+// its eliminated lines (blocked Gauss-Jordan, 64-wide panels) and six GEMMs
+// per kept line -- so the sequential depth is (levels x panels), about 100 for
+// 256 lines, instead of (lines x panels) for the Thomas recursion.  At level
+// 0 the couplings are the sparse CSR entries (a few per row), so the level-0
+// products are sparse-dense and cost O(m^2).  The factor (Y, P1, P2, L, U) is
+// kept in the storage type T (double, or float for the mixed-precision
+// variant, whose factorisation also runs in FP32).
+//
+// Solve: forward  r_k -= P1 r_e1 + P2 r_e2  (level by level),
+//        backward x_e = Y_e (r_e - L_e x_a - U_e x_b)  (levels in reverse).
+//
+// Replaces the sparse LU of the reference's direct solve
+// (scipy.sparse.linalg.splu, transiflow/interface/SciPy.py:131-162,194-258).
+#include "tf_common.h"
+
+#include <vector>
+
+namespace {
+
+constexpr int PB = 64;  // Gauss-Jordan panel width
+constexpr int KC = 8;   // couplings per row/column to a neighbouring line
+
+struct Cpl {
+  int idx;
+  int pad;
+  double val;
+};
+
+__global__ __launch_bounds__(256) void k_zero_int(long n, int* __restrict__ p) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
+}
+
+// zero n elements of each of a batch of matrices (grid.y = batch)
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_many(long n, T* const* __restrict__ ptrs) {
+  T* p = ptrs[blockIdx.y];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = (T)0;
+}
+
+// Row-wise couplings: Lr[r] (columns in line(r)-1, as offsets), Ur[r] (line(r)+1).
+// Column-wise couplings: Ut[c] = rows of line(c)-1 coupling into column c,
+//                        Lt[c] = rows of line(c)+1 coupling into column c.
+__global__ __launch_bounds__(256) void k_couplings4(int n, int m, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci, const double* __restrict__ v,
+                                                    int gauge, Cpl* __restrict__ Lr, int* __restrict__ nLr,
+                                                    Cpl* __restrict__ Ur, int* __restrict__ nUr,
+                                                    Cpl* __restrict__ Ut, int* __restrict__ nUt,
+                                                    Cpl* __restrict__ Lt, int* __restrict__ nLt,
+                                                    int* __restrict__ err) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int j = r / m;
+  int nl = 0, nu = 0;
+  if (r != gauge) {
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int c = ci[k];
+      if (c == gauge) continue;
+      const int jc = c / m;
+      if (jc == j) continue;
+      if (jc == j - 1) {
+        if (nl < KC) Lr[(long)r * KC + nl] = Cpl{c - jc * m, 0, v[k]};
+        else atomicOr(err, 1);
+        ++nl;
+        const int s = atomicAdd(&nLt[c], 1);
+        if (s < KC) Lt[(long)c * KC + s] = Cpl{r - j * m, 0, v[k]};
+        else atomicOr(err, 1);
+      } else if (jc == j + 1) {
+        if (nu < KC) Ur[(long)r * KC + nu] = Cpl{c - jc * m, 0, v[k]};
+        else atomicOr(err, 1);
+        ++nu;
+        const int s = atomicAdd(&nUt[c], 1);
+        if (s < KC) Ut[(long)c * KC + s] = Cpl{r - j * m, 0, v[k]};
+        else atomicOr(err, 1);
+      } else {
+        atomicOr(err, 2);
+      }
+    }
+  }
+  nLr[r] = nl < KC ? nl : KC;
+  nUr[r] = nu < KC ? nu : KC;
+}
+
+// Sort the column-wise lists by row (the atomic append order is arbitrary).
+__global__ __launch_bounds__(256) void k_sort_cols(int n, Cpl* __restrict__ A, int* __restrict__ cnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int k = cnt[c] < KC ? cnt[c] : KC;
+  cnt[c] = k;
+  Cpl* u = A + (long)c * KC;
+  for (int a = 1; a < k; ++a) {
+    Cpl key = u[a];
+    int b = a - 1;
+    while (b >= 0 && u[b].idx > key.idx) {
+      u[b + 1] = u[b];
+      --b;
+    }
+    u[b + 1] = key;
+  }
+}
+
+// D[j] = dense diagonal block of line j (gauged).  grid (m, N).
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_blocks(int m, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                     const double* __restrict__ v, int gauge, T* __restrict__ D) {
+  const int rl = blockIdx.x, j = blockIdx.y;
+  const long r = (long)j * m + rl;
+  T* row = D + ((long)j * m + rl) * m;
+  for (int c = threadIdx.x; c < m; c += blockDim.x) row[c] = (T)0;
+  __syncthreads();
+  if (r == gauge) {
+    if (threadIdx.x == 0) row[rl] = (T)-1;
+    return;
+  }
+  for (int k = rp[r] + threadIdx.x; k < rp[r + 1]; k += blockDim.x) {
+    const int c = ci[k];
+    if (c == gauge) continue;
+    const long cl = (long)c - (long)j * m;
+    if (cl >= 0 && cl < m) row[cl] = (T)v[k];
+  }
+}
+
+// ---- batched dense GEMM:  C_b = alpha A_b B_b + beta C_b  (row major) --------
+// 128 x 128 tiles, BK = 16, 256 threads, 8 x 8 outputs per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T* const* __restrict__ Ap, int lda,
+                                              T* const* __restrict__ Bp, int ldb, T* const* __restrict__ Cp, int ldc,
+                                              double alpha, double beta) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  __shared__ T As[BK][BM + 4];
+  __shared__ T Bs[BK][BN + 4];
+  const T* A = Ap[blockIdx.z];
+  const T* B = Bp[blockIdx.z];
+  T* C = Cp[blockIdx.z];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  T acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = (T)0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = t + q * 256;
+      const int mm = idx >> 4, kk = idx & 15;
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * lda + gk] : (T)0;
+      const int kb = idx >> 7, nn = idx & 127;
+      const int gk2 = k0 + kb, gn = n0 + nn;
+      Bs[kb][nn] = (gk2 < K && gn < N) ? B[(long)gk2 * ldb + gn] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; ++kk) {
+      T a[8], b[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[kk][ty * 4 + i];
+        a[4 + i] = As[kk][64 + ty * 4 + i];
+        b[i] = Bs[kk][tx * 4 + i];
+        b[4 + i] = Bs[kk][64 + tx * 4 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int gm = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + i - 4);
+    if (gm >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int gn = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
+      if (gn >= N) continue;
+      T* c = C + (long)gm * ldc + gn;
+      const double prev = beta != 0.0 ? (double)*c : 0.0;
+      *c = (T)(alpha * (double)acc[i][j] + beta * prev);
+    }
+  }
+}
+
+// ---- batched Gauss-Jordan inversion of m x m matrices ------------------------
+
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane);
+template <>
+__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+template <>
+__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Inverse of the pb x pb diagonal panel block of X_b at k0 (padded with the
+// identity to 64): Gauss-Jordan on [A | I], implicit partial pivoting; 8 waves,
+// lane = row, 16 augmented columns per wave in registers.  One block per matrix.
+template <typename T>
+__global__ __launch_bounds__(512) void k_panel(int m, int k0, int pb, T* const* __restrict__ Xp,
+                                               T* __restrict__ Pinv, int* __restrict__ nperturb) {
+  __shared__ T colk[2][64];
+  __shared__ T wmax[8];
+  const T* X = Xp[blockIdx.x];
+  T* P = Pinv + (long)blockIdx.x * PB * PB;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  T a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = wave * 16 + q;
+    T v;
+    if (c < 64)
+      v = (lane < pb && c < pb) ? X[(long)(k0 + lane) * m + k0 + c] : (lane == c ? (T)1 : (T)0);
+    else
+      v = (lane == c - 64) ? (T)1 : (T)0;
+    a[q] = v;
+  }
+  T mx = 0;
+  if (wave < 4) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(a[q]));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) wmax[wave] = mx;
+  __syncthreads();
+  T amax = 0;
+  for (int w = 0; w < 4; ++w) amax = fmax(amax, wmax[w]);
+  if (amax == (T)0) amax = 1;
+  const T tol = (sizeof(T) == 8 ? (T)1e-13 : (T)1e-6) * amax;
+  bool used = false;
+  int mystep = -1;
+  for (int k = 0; k < 64; ++k) {
+    if (wave == (k >> 4)) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q == (k & 15)) colk[k & 1][lane] = a[q];
+    }
+    __syncthreads();
+    const T f = colk[k & 1][lane];
+    // pivot: largest |f| among unused rows; 32-bit key (magnitude bits, lane) reduced with shuffles
+    // (unused rows get keys >= 64, ties go to the lower lane; used rows key 0)
+    unsigned key = 0;
+    if (!used) {
+      const unsigned bits = __float_as_uint((float)fabs(f));
+      key = (bits & ~63u) + 64u + (unsigned)(63 - lane);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned o = __shfl_xor(key, off, 64);
+      key = o > key ? o : key;
+    }
+    const int p = __builtin_amdgcn_readfirstlane(63 - (int)((key - 64u) & 63u));
+    T piv = readlane_t<T>(f, p);
+    if (fabs(piv) < tol) {
+      piv = piv < (T)0 ? -tol : tol;
+      if (threadIdx.x == 0) atomicAdd(nperturb, 1);
+    }
+    const T ip = (T)1 / piv;
+    if (lane == p) {
+      used = true;
+      mystep = k;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const T pr = readlane_t<T>(a[q], p) * ip;
+      a[q] = (lane == p) ? pr : a[q] - f * pr;
+    }
+  }
+  if (wave >= 4 && mystep >= 0 && mystep < pb) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = wave * 16 + q - 64;
+      if (c < pb) P[mystep * pb + c] = a[q];
+    }
+  }
+}
+
+// Snapshots for panel K = [k0, k0+pb) of X_b:
+//   Kr = X[K, :] with X[K, K] replaced by the identity   (pb x m)
+//   Kc = X[:, K] with the rows K zeroed                    (m x pb)
+// and X[:, K] is zeroed (the update rebuilds it).
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_copy(int m, int k0, int pb, T* const* __restrict__ Xp,
+                                                    T* __restrict__ Krb, T* __restrict__ Kcb) {
+  T* X = Xp[blockIdx.y];
+  T* Kr = Krb + (long)blockIdx.y * PB * m;
+  T* Kc = Kcb + (long)blockIdx.y * m * PB;
+  const long total = (long)pb * m;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int s = (int)(e / m), c = (int)(e % m);
+    const bool inK = c >= k0 && c < k0 + pb;
+    Kr[(long)s * m + c] = inK ? (c - k0 == s ? (T)1 : (T)0) : X[(long)(k0 + s) * m + c];
+  }
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / pb), s = (int)(e % pb);
+    const bool rowK = i >= k0 && i < k0 + pb;
+    T* x = X + (long)i * m + k0 + s;
+    Kc[(long)i * pb + s] = rowK ? (T)0 : *x;
+    *x = (T)0;
+  }
+}
+
+// X[K, :] = Tbuf  (after the rank-pb update)
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_store(int m, int k0, int pb, T* const* __restrict__ Xp,
+                                                     const T* __restrict__ Tb) {
+  T* X = Xp[blockIdx.y];
+  const T* Tm = Tb + (long)blockIdx.y * PB * m;
+  const long total = (long)pb * m;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+    X[(long)(k0 + e / m) * m + e % m] = Tm[e];
+}
+
+// ---- level-0 sparse products -----------------------------------------------
+
+// O_b[r][:] = sum_q cpl[rows of line_b][r][q].val * D_b[idx][:]      (sparse rows x dense)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp_rows(int m, const Cpl* __restrict__ cpl, const int* __restrict__ ncpl,
+                                                 const int* __restrict__ line, T* const* __restrict__ Dp,
+                                                 T* const* __restrict__ Op) {
+  const int rl = blockIdx.x, b = blockIdx.y;
+  const long r = (long)line[b] * m + rl;
+  const T* D = Dp[b];
+  T* O = Op[b] + (long)rl * m;
+  const int cnt = ncpl[r];
+  for (int c = threadIdx.x; c < m; c += blockDim.x) {
+    double acc = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+      const Cpl cp = cpl[r * KC + q];
+      acc += cp.val * (double)D[(long)cp.idx * m + c];
+    }
+    O[c] = (T)acc;
+  }
+}
+
+// O_b[r][c] = (acc ? O_b[r][c] : 0) + sign * sum_q P_b[r][cpl[col c of line_b].idx] * val   (dense x sparse cols)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ cpl, const int* __restrict__ ncpl,
+                                                 const int* __restrict__ line, T* const* __restrict__ Pp,
+                                                 T* const* __restrict__ Op, double sign, int accumulate) {
+  const int rl = blockIdx.x, b = blockIdx.y;
+  const T* P = Pp[b] + (long)rl * m;
+  T* O = Op[b] + (long)rl * m;
+  const long cbase = (long)line[b] * m;
+  for (int c = threadIdx.x; c < m; c += blockDim.x) {
+    const int cnt = ncpl[cbase + c];
+    double acc = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+      const Cpl cp = cpl[(cbase + c) * KC + q];
+      acc += (double)P[cp.idx] * cp.val;
+    }
+    O[c] = (T)((accumulate ? (double)O[c] : 0.0) + sign * acc);
+  }
+}
+
+// ---- solve ----------------------------------------------------------------------
+
+// rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines; one wave per row.
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_fwd(int m, const int* __restrict__ k_line, const int* __restrict__ e1,
+                                             const int* __restrict__ e2, T* const* __restrict__ P1p,
+                                             T* const* __restrict__ P2p, V* __restrict__ rw) {
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;
+  double acc = 0.0;
+  if (e1[b] >= 0) {
+    const T* p1 = P1p[b] + (long)r * m;
+    const V* x1 = rw + (long)e1[b] * m;
+    for (int c = lane; c < m; c += 64) acc += (double)p1[c] * (double)x1[c];
+  }
+  if (e2[b] >= 0) {
+    const T* p2 = P2p[b] + (long)r * m;
+    const V* x2 = rw + (long)e2[b] * m;
+    for (int c = lane; c < m; c += 64) acc += (double)p2[c] * (double)x2[c];
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) {
+    V* out = rw + (long)k_line[b] * m + r;
+    *out = (V)((double)*out - acc);
+  }
+}
+
+// t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, or the level-0 sparse rows)
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_bwd_t(int m, int sparse, const int* __restrict__ e_line,
+                                               const int* __restrict__ a_line, const int* __restrict__ b_line,
+                                               T* const* __restrict__ Lp, T* const* __restrict__ Up,
+                                               const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
+                                               const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
+                                               const V* __restrict__ rw, const V* __restrict__ x,
+                                               V* __restrict__ tbuf) {
+  const int bi = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;
+  const long row = (long)e_line[bi] * m + r;
+  double acc = 0.0;
+  const int a = a_line[bi], bb = b_line[bi];
+  if (sparse) {
+    if (lane == 0) {
+      if (a >= 0)
+        for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)x[(long)a * m + Lr[row * KC + q].idx];
+      if (bb >= 0)
+        for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
+    }
+  } else {
+    if (a >= 0) {
+      const T* l = Lp[bi] + (long)r * m;
+      for (int c = lane; c < m; c += 64) acc += (double)l[c] * (double)x[(long)a * m + c];
+    }
+    if (bb >= 0) {
+      const T* u = Up[bi] + (long)r * m;
+      for (int c = lane; c < m; c += 64) acc += (double)u[c] * (double)x[(long)bb * m + c];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  }
+  if (lane == 0) tbuf[row] = (V)((double)rw[row] - acc);
+}
+
+// x[e] = Y_e t[e]
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_bwd_x(int m, const int* __restrict__ e_line, T* const* __restrict__ Yp,
+                                               const V* __restrict__ tbuf, V* __restrict__ x) {
+  const int bi = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;
+  const long base = (long)e_line[bi] * m;
+  const T* y = Yp[bi] + (long)r * m;
+  double acc = 0.0;
+  for (int c = lane; c < m; c += 64) acc += (double)y[c] * (double)tbuf[base + c];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) x[base + r] = (V)acc;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void k_copy(long n, const V* __restrict__ a, V* __restrict__ b) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) b[e] = a[e];
+}
+
+// ---- host-side schedule ----------------------------------------------------------
+
+struct LevelPlan {
+  // eliminated lines
+  std::vector<int> e, ea, eb;
+  // kept lines
+  std::vector<int> k, k1, k2, kfa, kfb, kslot;
+};
+
+std::vector<LevelPlan> make_plan(int N) {
+  // The last active line is always kept (it is eliminated alone at the final
+  // level): with a no-slip lid its raw diagonal block carries the 1D pressure
+  // null space and is singular until the lines below it have been folded in.
+  std::vector<LevelPlan> plan;
+  std::vector<int> A(N);
+  for (int j = 0; j < N; ++j) A[j] = j;
+  int slot = 0;
+  while (!A.empty()) {
+    LevelPlan L;
+    const int na = (int)A.size();
+    const int kp = na > 1 ? (na - 1) % 2 : 1;  // parity of the kept positions
+    for (int p = 0; p < na; ++p) {
+      if (p % 2 != kp) {
+        L.e.push_back(A[p]);
+        L.ea.push_back(p >= 1 ? A[p - 1] : -1);
+        L.eb.push_back(p + 1 < na ? A[p + 1] : -1);
+      } else {
+        L.k.push_back(A[p]);
+        L.k1.push_back(p >= 1 ? A[p - 1] : -1);
+        L.k2.push_back(p + 1 < na ? A[p + 1] : -1);
+        L.kfa.push_back(p >= 2 ? A[p - 2] : -1);
+        L.kfb.push_back(p + 2 < na ? A[p + 2] : -1);
+        L.kslot.push_back(slot++);
+      }
+    }
+    plan.push_back(L);
+    A = L.k;
+  }
+  return plan;
+}
+
+}  // namespace
+
+// Device-side pointer/index lists of one level, all in one pool.
+struct BcrLevelDev {
+  int ne, nk;
+  int *e, *ea, *eb;        // eliminated lines and their kept neighbours (-1: none)
+  void **Ye, **Le, **Ue;   // their D (inverted in place), L, U
+  int *k, *k1, *k2;        // kept lines and their eliminated neighbours (-1: none)
+  void **P1, **P2;         // per kept line
+  // operand lists of the sub-batches (kept lines with e1 / with e2 / with a line before e1 / after e2)
+  int n_p1, n_p2, n_fa, n_fb, n_nofa, n_nofb;
+  int *k_hase1_line, *k_hase2_line, *k_fa_line, *k_fb_line;
+  void **g_p1_A, **g_p1_B, **g_p1_C;  // P1 = L_k Y_e1
+  void **g_d1_A, **g_d1_B;            // D_k -= P1 U_e1
+  void **g_p2_A, **g_p2_B, **g_p2_C;  // P2 = U_k Y_e2
+  void **g_d2_A, **g_d2_B;            // D_k -= P2 L_e2
+  void **g_fa_A, **g_fa_B, **g_fa_C;  // L_k = -P1 L_e1
+  void **g_fb_A, **g_fb_B, **g_fb_C;  // U_k = -P2 U_e2
+  void **z_fa, **z_fb;                // L_k (U_k) that become zero
+};
+
+struct tf_bcr {
+  int n, m, N, single;
+  size_t es;
+  void* D;    // N m^2: diagonal blocks, inverted in place (Y)
+  void* Lm;   // N m^2: dense lower couplings (levels >= 1)
+  void* Um;   // N m^2
+  void* P1;   // slots m^2 (one per kept line per level)
+  void* P2;   // slots m^2
+  void* Pinv; // batch x 64 x 64
+  void* Kr;   // batch x 64 x m
+  void* Kc;   // batch x m x 64
+  void* Tb;   // batch x 64 x m
+  Cpl *Lr, *Ur, *Ut, *Lt;
+  int *nLr, *nUr, *nUt, *nLt;
+  int* flags;  // [err, nperturb]
+  int* host_flags;
+  void* rw;    // n (solve workspace)
+  void* tb;    // n
+  void* pool;  // device pointer/index pool
+  std::vector<LevelPlan> plan;
+  std::vector<BcrLevelDev> lev;
+};
+
+namespace {
+
+template <typename T>
+void gemm(int M, int N, int K, void** A, int lda, void** B, int ldb, void** C, int ldc, double alpha, double beta,
+          int batch, hipStream_t st) {
+  if (batch <= 0) return;
+  TF_PROF_BEGIN(tf::PROF_GEMM, st);
+  hipLaunchKernelGGL(k_gemm<T>, dim3((N + 127) / 128, (M + 127) / 128, batch), dim3(256), 0, st, M, N, K,
+                     (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
+  TF_PROF_END(tf::PROF_GEMM, st);
+}
+
+template <typename T>
+void invert_batch(tf_bcr* h, void** Xp, int batch, hipStream_t st) {
+  const int m = h->m;
+  T* Pinv = (T*)h->Pinv;
+  T* Kr = (T*)h->Kr;
+  T* Kc = (T*)h->Kc;
+  T* Tb = (T*)h->Tb;
+  // per-batch pointers to the snapshot buffers are laid out contiguously: the
+  // GEMMs need pointer lists, which live right after Xp in the pool
+  void** pinvp = Xp + batch;
+  void** krp = pinvp + batch;
+  void** kcp = krp + batch;
+  void** tbp = kcp + batch;
+  const long sz = (long)PB * m;
+  const int gcopy = (int)((sz + 255) / 256) < 1024 ? (int)((sz + 255) / 256) : 1024;
+  for (int k0 = 0; k0 < m; k0 += PB) {
+    const int pb = m - k0 < PB ? m - k0 : PB;
+    TF_PROF_BEGIN(tf::PROF_GJ_PANEL, st);
+    hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, st, m, k0, pb, (T* const*)Xp, Pinv, h->flags + 1);
+    TF_PROF_END(tf::PROF_GJ_PANEL, st);
+    hipLaunchKernelGGL(k_panel_copy<T>, dim3(gcopy, batch), dim3(256), 0, st, m, k0, pb, (T* const*)Xp, Kr, Kc);
+    // T = Pinv * Kr   (pb x pb) x (pb x m)
+    gemm<T>(pb, m, pb, pinvp, pb, krp, m, tbp, m, 1.0, 0.0, batch, st);
+    // X -= Kc * T     (m x pb) x (pb x m)
+    TF_PROF_BEGIN(tf::PROF_GJ_UPDATE, st);
+    gemm<T>(m, m, pb, kcp, pb, tbp, m, Xp, m, -1.0, 1.0, batch, st);
+    TF_PROF_END(tf::PROF_GJ_UPDATE, st);
+    hipLaunchKernelGGL(k_panel_store<T>, dim3(gcopy, batch), dim3(256), 0, st, m, k0, pb, (T* const*)Xp,
+                       (const T*)Tb);
+  }
+  (void)Pinv; (void)Kr; (void)Kc; (void)Tb;
+}
+
+template <typename T>
+int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int gauge, hipStream_t st) {
+  const int m = h->m, N = h->N, n = h->n;
+  hipLaunchKernelGGL(k_zero_int, dim3(4), dim3(256), 0, st, 2L, h->flags);
+  hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (long)n, h->nUt);
+  hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (long)n, h->nLt);
+  hipLaunchKernelGGL(k_couplings4, dim3((n + 255) / 256), dim3(256), 0, st, n, m, rp, ci, v, gauge, h->Lr, h->nLr,
+                     h->Ur, h->nUr, h->Ut, h->nUt, h->Lt, h->nLt, h->flags);
+  hipLaunchKernelGGL(k_sort_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, h->Ut, h->nUt);
+  hipLaunchKernelGGL(k_sort_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, h->Lt, h->nLt);
+  hipLaunchKernelGGL(k_diag_blocks<T>, dim3(m, N), dim3(256), 0, st, m, rp, ci, v, gauge, (T*)h->D);
+  for (size_t l = 0; l < h->lev.size(); ++l) {
+    BcrLevelDev& L = h->lev[l];
+    // 1. invert the eliminated lines' diagonal blocks in place
+    invert_batch<T>(h, L.Ye, L.ne, st);
+    if (L.nk == 0) continue;
+    if (l == 0) {
+      // sparse couplings at level 0 (e1 = k-1, e2 = k+1):
+      // P1 = L_k Y_e1, P2 = U_k Y_e2 (sparse rows of line k)
+      if (L.n_p1)
+        hipLaunchKernelGGL(k_sp_rows<T>, dim3(m, L.n_p1), dim3(256), 0, st, m, h->Lr, h->nLr, L.k_hase1_line,
+                           (T* const*)L.g_p1_B, (T* const*)L.g_p1_C);
+      if (L.n_p2)
+        hipLaunchKernelGGL(k_sp_rows<T>, dim3(m, L.n_p2), dim3(256), 0, st, m, h->Ur, h->nUr, L.k_hase2_line,
+                           (T* const*)L.g_p2_B, (T* const*)L.g_p2_C);
+      // D_k -= P1 U_e1 (rows of line k-1 into the columns of line k: Ut)
+      // D_k -= P2 L_e2 (rows of line k+1 into the columns of line k: Lt)
+      if (L.n_p1)
+        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_p1), dim3(256), 0, st, m, h->Ut, h->nUt, L.k_hase1_line,
+                           (T* const*)L.g_p1_C, (T* const*)L.g_d1_A, -1.0, 1);
+      if (L.n_p2)
+        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_p2), dim3(256), 0, st, m, h->Lt, h->nLt, L.k_hase2_line,
+                           (T* const*)L.g_p2_C, (T* const*)L.g_d2_A, -1.0, 1);
+      // L_k = -P1 L_e1 (columns of line k-2: Lt) ; U_k = -P2 U_e2 (columns of line k+2: Ut)
+      if (L.n_fa)
+        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_fa), dim3(256), 0, st, m, h->Lt, h->nLt, L.k_fa_line,
+                           (T* const*)L.g_fa_A, (T* const*)L.g_fa_C, -1.0, 0);
+      if (L.n_fb)
+        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_fb), dim3(256), 0, st, m, h->Ut, h->nUt, L.k_fb_line,
+                           (T* const*)L.g_fb_A, (T* const*)L.g_fb_C, -1.0, 0);
+    } else {
+      gemm<T>(m, m, m, L.g_p1_A, m, L.g_p1_B, m, L.g_p1_C, m, 1.0, 0.0, L.n_p1, st);
+      gemm<T>(m, m, m, L.g_p2_A, m, L.g_p2_B, m, L.g_p2_C, m, 1.0, 0.0, L.n_p2, st);
+      gemm<T>(m, m, m, L.g_p1_C, m, L.g_d1_B, m, L.g_d1_A, m, -1.0, 1.0, L.n_p1, st);
+      gemm<T>(m, m, m, L.g_p2_C, m, L.g_d2_B, m, L.g_d2_A, m, -1.0, 1.0, L.n_p2, st);
+      gemm<T>(m, m, m, L.g_fa_A, m, L.g_fa_B, m, L.g_fa_C, m, -1.0, 0.0, L.n_fa, st);
+      gemm<T>(m, m, m, L.g_fb_A, m, L.g_fb_B, m, L.g_fb_C, m, -1.0, 0.0, L.n_fb, st);
+    }
+    // couplings that no longer exist
+    const long mm = (long)m * m;
+    if (L.n_nofa)
+      hipLaunchKernelGGL(k_zero_many<T>, dim3(256, L.n_nofa), dim3(256), 0, st, mm, (T* const*)L.z_fa);
+    if (L.n_nofb)
+      hipLaunchKernelGGL(k_zero_many<T>, dim3(256, L.n_nofb), dim3(256), 0, st, mm, (T* const*)L.z_fb);
+  }
+  TF_LAUNCH_CHECK("bcr factor");
+  return 0;
+}
+
+template <typename T, typename V>
+int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
+  const int m = h->m;
+  V* rw = (V*)h->rw;
+  V* tb = (V*)h->tb;
+  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
+  const dim3 blk(256);
+  for (size_t l = 0; l < h->lev.size(); ++l) {
+    BcrLevelDev& L = h->lev[l];
+    if (L.nk)
+      hipLaunchKernelGGL((k_fwd<T, V>), dim3((m + 3) / 4, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, (T* const*)L.P1,
+                         (T* const*)L.P2, rw);
+  }
+  for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
+    BcrLevelDev& L = h->lev[l];
+    hipLaunchKernelGGL((k_bwd_t<T, V>), dim3((m + 3) / 4, L.ne), blk, 0, st, m, l == 0 ? 1 : 0, L.e, L.ea, L.eb,
+                       (T* const*)L.Le, (T* const*)L.Ue, h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)x, tb);
+    hipLaunchKernelGGL((k_bwd_x<T, V>), dim3((m + 3) / 4, L.ne), blk, 0, st, m, L.e, (T* const*)L.Ye, (const V*)tb,
+                       x);
+  }
+  TF_LAUNCH_CHECK("bcr solve");
+  return 0;
+}
+
+}  // namespace
+
+// ---- C ABI -----------------------------------------------------------------------
+
+namespace {
+
+struct PoolBuilder {
+  std::vector<char> bytes;
+  // returns the byte offset of n pointer-sized (or int) slots
+  size_t add_ptrs(const std::vector<void*>& v) {
+    size_t off = align();
+    bytes.resize(off + v.size() * sizeof(void*));
+    if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(void*));
+    return off;
+  }
+  size_t add_ints(const std::vector<int>& v) {
+    size_t off = align();
+    bytes.resize(off + v.size() * sizeof(int));
+    if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(int));
+    return off;
+  }
+  size_t align() {
+    size_t o = (bytes.size() + 63) & ~size_t(63);
+    bytes.resize(o);
+    return o;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int tf_bcr_create(tf_bcr** out, int n, int m, int single) {
+  if (!out) return tf::fail("null handle");
+  *out = nullptr;
+  if (m < 1 || n % m != 0) return tf::fail("line block size must divide n");
+  tf_bcr* h = new tf_bcr();
+  h->n = n;
+  h->m = m;
+  h->N = n / m;
+  h->single = single;
+  h->es = single ? sizeof(float) : sizeof(double);
+  const size_t mm = (size_t)m * m, N = h->N;
+  const size_t maxb = (N + 1) / 2;
+  TF_HIP(hipMalloc(&h->D, h->es * mm * N));
+  TF_HIP(hipMalloc(&h->Lm, h->es * mm * N));
+  TF_HIP(hipMalloc(&h->Um, h->es * mm * N));
+  // one multiplier pair per kept line per level (more than N - 1 when the
+  // active count is odd at some level)
+  h->plan = make_plan(h->N);
+  size_t slots = 0;
+  for (const LevelPlan& L : h->plan) slots += L.k.size();
+  if (slots < 1) slots = 1;
+  TF_HIP(hipMalloc(&h->P1, h->es * mm * slots));
+  TF_HIP(hipMalloc(&h->P2, h->es * mm * slots));
+  TF_HIP(hipMalloc(&h->Pinv, h->es * PB * PB * maxb));
+  TF_HIP(hipMalloc(&h->Kr, h->es * PB * (size_t)m * maxb));
+  TF_HIP(hipMalloc(&h->Kc, h->es * PB * (size_t)m * maxb));
+  TF_HIP(hipMalloc(&h->Tb, h->es * PB * (size_t)m * maxb));
+  for (Cpl** c : {&h->Lr, &h->Ur, &h->Ut, &h->Lt}) TF_HIP(hipMalloc(c, sizeof(Cpl) * KC * (size_t)n));
+  for (int** c : {&h->nLr, &h->nUr, &h->nUt, &h->nLt}) TF_HIP(hipMalloc(c, sizeof(int) * (size_t)n));
+  TF_HIP(hipMalloc(&h->flags, sizeof(int) * 2));
+  TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
+  TF_HIP(hipMalloc(&h->rw, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->tb, sizeof(double) * (size_t)n));
+
+  // device pointer lists of the schedule
+  char* D = (char*)h->D;
+  char* Lm = (char*)h->Lm;
+  char* Um = (char*)h->Um;
+  char* P1 = (char*)h->P1;
+  char* P2 = (char*)h->P2;
+  const size_t bm = h->es * mm;
+  auto blk = [&](char* base, int j) -> void* { return (void*)(base + bm * (size_t)j); };
+  PoolBuilder pb;
+  struct Off {
+    size_t e, ea, eb, Ye, Le, Ue, k, k1, k2, P1, P2, p1A, p1B, p1C, d1A, d1B, p2A, p2B, p2C, d2A, d2B, faA, faB,
+        faC, fbA, fbB, fbC, zfa, zfb, hase1, hase2, faline, fbline;
+    int ne, nk, np1, np2, nfa, nfb, nnofa, nnofb;
+  };
+  std::vector<Off> offs;
+  for (const LevelPlan& L : h->plan) {
+    Off o{};
+    o.ne = (int)L.e.size();
+    o.nk = (int)L.k.size();
+    o.e = pb.add_ints(L.e);
+    o.ea = pb.add_ints(L.ea);
+    o.eb = pb.add_ints(L.eb);
+    std::vector<void*> ye, le, ue, pinv, kr, kc, tbv;
+    for (int b = 0; b < o.ne; ++b) {
+      ye.push_back(blk(D, L.e[b]));
+      le.push_back(blk(Lm, L.e[b]));
+      ue.push_back(blk(Um, L.e[b]));
+      pinv.push_back((char*)h->Pinv + h->es * PB * PB * b);
+      kr.push_back((char*)h->Kr + h->es * PB * (size_t)m * b);
+      kc.push_back((char*)h->Kc + h->es * PB * (size_t)m * b);
+      tbv.push_back((char*)h->Tb + h->es * PB * (size_t)m * b);
+    }
+    // Ye must be followed by the snapshot pointer lists (see invert_batch)
+    std::vector<void*> ychain = ye;
+    for (auto* v : {&pinv, &kr, &kc, &tbv}) ychain.insert(ychain.end(), v->begin(), v->end());
+    o.Ye = pb.add_ptrs(ychain);
+    o.Le = pb.add_ptrs(le);
+    o.Ue = pb.add_ptrs(ue);
+    o.k = pb.add_ints(L.k);
+    o.k1 = pb.add_ints(L.k1);
+    o.k2 = pb.add_ints(L.k2);
+    std::vector<void*> p1, p2;
+    std::vector<void*> p1A, p1B, p1C, d1A, d1B, p2A, p2B, p2C, d2A, d2B, faA, faB, faC, fbA, fbB, fbC, zfa, zfb;
+    std::vector<int> hase1, hase2, faline, fbline;
+    for (int b = 0; b < o.nk; ++b) {
+      const int k = L.k[b], e1 = L.k1[b], e2 = L.k2[b], s = L.kslot[b];
+      p1.push_back(blk(P1, s));
+      p2.push_back(blk(P2, s));
+      if (e1 >= 0) {
+        p1A.push_back(blk(Lm, k));
+        p1B.push_back(blk(D, e1));
+        p1C.push_back(blk(P1, s));
+        d1A.push_back(blk(D, k));
+        d1B.push_back(blk(Um, e1));
+        hase1.push_back(k);
+      }
+      if (e2 >= 0) {
+        p2A.push_back(blk(Um, k));
+        p2B.push_back(blk(D, e2));
+        p2C.push_back(blk(P2, s));
+        d2A.push_back(blk(D, k));
+        d2B.push_back(blk(Lm, e2));
+        hase2.push_back(k);
+      }
+      if (e1 >= 0 && L.kfa[b] >= 0) {
+        faA.push_back(blk(P1, s));
+        faB.push_back(blk(Lm, e1));
+        faC.push_back(blk(Lm, k));
+        faline.push_back(L.kfa[b]);
+      } else {
+        zfa.push_back(blk(Lm, k));
+      }
+      if (e2 >= 0 && L.kfb[b] >= 0) {
+        fbA.push_back(blk(P2, s));
+        fbB.push_back(blk(Um, e2));
+        fbC.push_back(blk(Um, k));
+        fbline.push_back(L.kfb[b]);
+      } else {
+        zfb.push_back(blk(Um, k));
+      }
+    }
+    o.P1 = pb.add_ptrs(p1);
+    o.P2 = pb.add_ptrs(p2);
+    o.np1 = (int)p1A.size();
+    o.p1A = pb.add_ptrs(p1A);
+    o.p1B = pb.add_ptrs(p1B);
+    o.p1C = pb.add_ptrs(p1C);
+    o.d1A = pb.add_ptrs(d1A);
+    o.d1B = pb.add_ptrs(d1B);
+    o.hase1 = pb.add_ints(hase1);
+    o.np2 = (int)p2A.size();
+    o.p2A = pb.add_ptrs(p2A);
+    o.p2B = pb.add_ptrs(p2B);
+    o.p2C = pb.add_ptrs(p2C);
+    o.d2A = pb.add_ptrs(d2A);
+    o.d2B = pb.add_ptrs(d2B);
+    o.hase2 = pb.add_ints(hase2);
+    o.nfa = (int)faA.size();
+    o.faA = pb.add_ptrs(faA);
+    o.faB = pb.add_ptrs(faB);
+    o.faC = pb.add_ptrs(faC);
+    o.faline = pb.add_ints(faline);
+    o.nfb = (int)fbA.size();
+    o.fbA = pb.add_ptrs(fbA);
+    o.fbB = pb.add_ptrs(fbB);
+    o.fbC = pb.add_ptrs(fbC);
+    o.fbline = pb.add_ints(fbline);
+    o.nnofa = (int)zfa.size();
+    o.zfa = pb.add_ptrs(zfa);
+    o.nnofb = (int)zfb.size();
+    o.zfb = pb.add_ptrs(zfb);
+    offs.push_back(o);
+  }
+  pb.align();
+  TF_HIP(hipMalloc(&h->pool, pb.bytes.size() + 64));
+  TF_HIP(hipMemcpy(h->pool, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice));
+  char* P = (char*)h->pool;
+  for (const Off& o : offs) {
+    BcrLevelDev L{};
+    L.ne = o.ne;
+    L.nk = o.nk;
+    L.e = (int*)(P + o.e);
+    L.ea = (int*)(P + o.ea);
+    L.eb = (int*)(P + o.eb);
+    L.Ye = (void**)(P + o.Ye);
+    L.Le = (void**)(P + o.Le);
+    L.Ue = (void**)(P + o.Ue);
+    L.k = (int*)(P + o.k);
+    L.k1 = (int*)(P + o.k1);
+    L.k2 = (int*)(P + o.k2);
+    L.P1 = (void**)(P + o.P1);
+    L.P2 = (void**)(P + o.P2);
+    L.n_p1 = o.np1;
+    L.g_p1_A = (void**)(P + o.p1A);
+    L.g_p1_B = (void**)(P + o.p1B);
+    L.g_p1_C = (void**)(P + o.p1C);
+    L.g_d1_A = (void**)(P + o.d1A);
+    L.g_d1_B = (void**)(P + o.d1B);
+    L.k_hase1_line = (int*)(P + o.hase1);
+    L.n_p2 = o.np2;
+    L.g_p2_A = (void**)(P + o.p2A);
+    L.g_p2_B = (void**)(P + o.p2B);
+    L.g_p2_C = (void**)(P + o.p2C);
+    L.g_d2_A = (void**)(P + o.d2A);
+    L.g_d2_B = (void**)(P + o.d2B);
+    L.k_hase2_line = (int*)(P + o.hase2);
+    L.n_fa = o.nfa;
+    L.g_fa_A = (void**)(P + o.faA);
+    L.g_fa_B = (void**)(P + o.faB);
+    L.g_fa_C = (void**)(P + o.faC);
+    L.k_fa_line = (int*)(P + o.faline);
+    L.n_fb = o.nfb;
+    L.g_fb_A = (void**)(P + o.fbA);
+    L.g_fb_B = (void**)(P + o.fbB);
+    L.g_fb_C = (void**)(P + o.fbC);
+    L.k_fb_line = (int*)(P + o.fbline);
+    L.n_nofa = o.nnofa;
+    L.z_fa = (void**)(P + o.zfa);
+    L.n_nofb = o.nnofb;
+    L.z_fb = (void**)(P + o.zfb);
+    h->lev.push_back(L);
+  }
+  *out = h;
+  return 0;
+}
+
+int tf_bcr_destroy(tf_bcr* h) {
+  if (!h) return 0;
+  for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
+                  (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
+                  (void*)h->flags, h->rw, h->tb, h->pool})
+    (void)hipFree(p);
+  (void)hipHostFree(h->host_flags);
+  delete h;
+  return 0;
+}
+
+// Factor the gauged matrix (CSR, gauge row or -1).  Asynchronous on ``stream``.
+int tf_bcr_factor(tf_bcr* h, const int* rp, const int* ci, const double* v, int gauge, void* stream) {
+  if (!h) return tf::fail("null handle");
+  hipStream_t st = (hipStream_t)stream;
+  return h->single ? factor_impl<float>(h, rp, ci, v, gauge, st) : factor_impl<double>(h, rp, ci, v, gauge, st);
+}
+
+// flags[0]: structure errors (1 too many couplings, 2 coupling beyond the next line);
+// flags[1]: statically perturbed pivots.  Synchronises the stream.
+int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream) {
+  if (!h || !flags_host) return tf::fail("null argument");
+  hipStream_t st = (hipStream_t)stream;
+  TF_HIP(hipMemcpyAsync(h->host_flags, h->flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+  TF_HIP(hipStreamSynchronize(st));
+  flags_host[0] = h->host_flags[0];
+  flags_host[1] = h->host_flags[1];
+  return 0;
+}
+
+int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream) {
+  if (!h) return tf::fail("null handle");
+  hipStream_t st = (hipStream_t)stream;
+  return h->single ? solve_impl<float, double>(h, r, x, st) : solve_impl<double, double>(h, r, x, st);
+}
+
+}  // extern "C"

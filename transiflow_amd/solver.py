@@ -10,8 +10,10 @@ reference's splu solve is matched to that relative residual).
 Options (``parameters['HIP Solver']``, all optional):
 
 ``'Preconditioner'``   ``'line'`` (exact block-tridiagonal factorisation over
-                       grid lines, default when a grid line fits the memory
-                       budget), ``'block-jacobi'`` (cell blocks);
+                       grid lines by block cyclic reduction, default when the
+                       dense line blocks fit the memory budget), ``'thomas'``
+                       (the same factorisation by the sequential block Thomas
+                       recursion), ``'block-jacobi'`` (cell blocks);
 ``'Precision'``        ``'fp64'`` (default) or ``'mixed'``: the preconditioner is
                        stored in FP32 and reused across Newton iterations while
                        GMRES converges quickly; GMRES and the residual stay FP64;
@@ -71,36 +73,40 @@ class CellBlockJacobi:
 
 
 class LineFactor:
-    """Exact block-tridiagonal factorisation over grid lines (see csrc/linelu.hip)."""
-    name = 'line'
+    """Exact factorisation of the gauged Jacobian, block tridiagonal over grid
+    lines: block cyclic reduction (``'line'``, csrc/bcr.hip) or the sequential
+    Thomas recursion (``'thomas'``, csrc/linelu.hip)."""
+    _ABI = {'line': 'tf_bcr', 'thomas': 'tf_linelu'}
 
-    def __init__(self, itf, jac, gauge, single=False):
+    def __init__(self, itf, jac, gauge, single=False, kind='line'):
         m = line_block_size(itf)
+        self.name = kind
+        self._p = self._ABI[kind]
         h = ctypes.c_void_p()
-        _lib.call('tf_linelu_create', ctypes.byref(h), itf.n, m, 1 if single else 0)
+        _lib.call(self._p + '_create', ctypes.byref(h), itf.n, m, 1 if single else 0)
         self._h = h
         self.single = single
         self.m = m
         self.refactor(jac, gauge)
 
     def refactor(self, jac, gauge):
-        _lib.call('tf_linelu_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), stream())
+        _lib.call(self._p + '_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), stream())
         flags = (ctypes.c_int * 2)()
-        _lib.call('tf_linelu_status', self._h, flags, stream())
+        _lib.call(self._p + '_status', self._h, flags, stream())
         if flags[0]:
             raise SolverError('line factorisation: matrix is not block tridiagonal over grid lines (flags %d)'
                               % flags[0])
         self.perturbed = flags[1]
 
     def apply(self, r, out):
-        _lib.call('tf_linelu_solve_f64', self._h, ptr(r), ptr(out), stream())
+        _lib.call(self._p + '_solve_f64', self._h, ptr(r), ptr(out), stream())
         return out
 
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and h.value:
             try:
-                _lib.load().tf_linelu_destroy(h)
+                getattr(_lib.load(), self._p + '_destroy')(h)
             except Exception:
                 pass
             self._h = None
@@ -114,9 +120,11 @@ def line_block_size(itf):
 
 
 def line_fits(itf, single=False):
+    """Block cyclic reduction keeps five dense m x m blocks per line (D, L, U and
+    the two elimination multipliers)."""
     m = line_block_size(itf)
     nlines = itf.n // m
-    return 3 * nlines * m * m * (4 if single else 8) <= LINE_BUDGET_BYTES
+    return 5 * nlines * m * m * (4 if single else 8) <= LINE_BUDGET_BYTES
 
 
 class Krylov:
@@ -241,8 +249,8 @@ class Solver:
         g = self.gauge()
         if kind == 'auto':
             kind = 'line' if line_fits(self.itf, single) else 'block-jacobi'
-        if kind == 'line':
-            return LineFactor(self.itf, jac, g, single=single)
+        if kind in ('line', 'thomas'):
+            return LineFactor(self.itf, jac, g, single=single, kind=kind)
         if kind == 'block-jacobi':
             return CellBlockJacobi(self.itf, jac, g)
         raise ValueError('unknown preconditioner %r' % kind)
