@@ -145,6 +145,7 @@ int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
 int tf_prof_enable(int on);
 int tf_prof_read(int id, long* count, double* ms_total);
 int tf_prof_reset(void);
+int tf_prof_work(int id, double* work_total);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,7 @@ SIGNATURES = {
     'tf_prof_enable': (_c_int, [_c_int]),
     'tf_prof_read': (_c_int, [_c_int, ctypes.POINTER(_c_long), _dp]),
     'tf_prof_reset': (_c_int, []),
+    'tf_prof_work': (_c_int, [_c_int, ctypes.POINTER(ctypes.c_double)]),
     'tf_problem_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int,
                                    _dp, _dp, _dp, _c_int, _c_int, _c_int]),
     'tf_problem_destroy': (_c_int, [_vp]),

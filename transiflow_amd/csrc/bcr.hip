@@ -29,6 +29,7 @@
 //
 // Replaces the sparse LU of the reference's direct solve
 // (scipy.sparse.linalg.splu, transiflow/interface/SciPy.py:131-162,194-258).
+#include "gemm.h"
 #include "tf_common.h"
 
 #include <vector>
@@ -134,69 +135,6 @@ __global__ __launch_bounds__(256) void k_diag_blocks(int m, const int* __restric
     if (c == gauge) continue;
     const long cl = (long)c - (long)j * m;
     if (cl >= 0 && cl < m) row[cl] = (T)v[k];
-  }
-}
-
-// ---- batched dense GEMM:  C_b = alpha A_b B_b + beta C_b  (row major) --------
-// 128 x 128 tiles, BK = 16, 256 threads, 8 x 8 outputs per thread.
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, T* const* __restrict__ Ap, int lda,
-                                              T* const* __restrict__ Bp, int ldb, T* const* __restrict__ Cp, int ldc,
-                                              double alpha, double beta) {
-  constexpr int BM = 128, BN = 128, BK = 16;
-  __shared__ T As[BK][BM + 4];
-  __shared__ T Bs[BK][BN + 4];
-  const T* A = Ap[blockIdx.z];
-  const T* B = Bp[blockIdx.z];
-  T* C = Cp[blockIdx.z];
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
-  T acc[8][8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) acc[i][j] = (T)0;
-  for (int k0 = 0; k0 < K; k0 += BK) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int idx = t + q * 256;
-      const int mm = idx >> 4, kk = idx & 15;
-      const int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? A[(long)gm * lda + gk] : (T)0;
-      const int kb = idx >> 7, nn = idx & 127;
-      const int gk2 = k0 + kb, gn = n0 + nn;
-      Bs[kb][nn] = (gk2 < K && gn < N) ? B[(long)gk2 * ldb + gn] : (T)0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; ++kk) {
-      T a[8], b[8];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[i] = As[kk][ty * 4 + i];
-        a[4 + i] = As[kk][64 + ty * 4 + i];
-        b[i] = Bs[kk][tx * 4 + i];
-        b[4 + i] = Bs[kk][64 + tx * 4 + i];
-      }
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] += a[i] * b[j];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int gm = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + i - 4);
-    if (gm >= M) continue;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int gn = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
-      if (gn >= N) continue;
-      T* c = C + (long)gm * ldc + gn;
-      const double prev = beta != 0.0 ? (double)*c : 0.0;
-      *c = (T)(alpha * (double)acc[i][j] + beta * prev);
-    }
   }
 }
 
@@ -545,6 +483,7 @@ struct tf_bcr {
   void* pool;  // device pointer/index pool
   std::vector<LevelPlan> plan;
   std::vector<BcrLevelDev> lev;
+  size_t solve_bytes;  // algorithmic bytes of one solve (dense blocks read once, vectors)
 };
 
 namespace {
@@ -553,9 +492,16 @@ template <typename T>
 void gemm(int M, int N, int K, void** A, int lda, void** B, int ldb, void** C, int ldc, double alpha, double beta,
           int batch, hipStream_t st) {
   if (batch <= 0) return;
+  TF_PROF_WORK(tf::PROF_GEMM, 2.0 * M * N * K * batch);
   TF_PROF_BEGIN(tf::PROF_GEMM, st);
-  hipLaunchKernelGGL(k_gemm<T>, dim3((N + 127) / 128, (M + 127) / 128, batch), dim3(256), 0, st, M, N, K,
-                     (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
+  // 128 x 128 tiles when they alone fill the chip, 64 x 64 otherwise
+  const long big = (long)((N + 127) / 128) * ((M + 127) / 128) * batch;
+  if (big >= 512)
+    hipLaunchKernelGGL((tf::k_gemm_mfma<T, 128, 128>), dim3((N + 127) / 128, (M + 127) / 128, batch), dim3(256), 0,
+                       st, M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
+  else
+    hipLaunchKernelGGL((tf::k_gemm_mfma<T, 64, 64>), dim3((N + 63) / 64, (M + 63) / 64, batch), dim3(256), 0, st,
+                       M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
   TF_PROF_END(tf::PROF_GEMM, st);
 }
 
@@ -576,6 +522,7 @@ void invert_batch(tf_bcr* h, void** Xp, int batch, hipStream_t st) {
   const int gcopy = (int)((sz + 255) / 256) < 1024 ? (int)((sz + 255) / 256) : 1024;
   for (int k0 = 0; k0 < m; k0 += PB) {
     const int pb = m - k0 < PB ? m - k0 : PB;
+    TF_PROF_WORK(tf::PROF_GJ_PANEL, 2.0 * 64 * 64 * 128 * batch);
     TF_PROF_BEGIN(tf::PROF_GJ_PANEL, st);
     hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, st, m, k0, pb, (T* const*)Xp, Pinv, h->flags + 1);
     TF_PROF_END(tf::PROF_GJ_PANEL, st);
@@ -583,9 +530,7 @@ void invert_batch(tf_bcr* h, void** Xp, int batch, hipStream_t st) {
     // T = Pinv * Kr   (pb x pb) x (pb x m)
     gemm<T>(pb, m, pb, pinvp, pb, krp, m, tbp, m, 1.0, 0.0, batch, st);
     // X -= Kc * T     (m x pb) x (pb x m)
-    TF_PROF_BEGIN(tf::PROF_GJ_UPDATE, st);
     gemm<T>(m, m, pb, kcp, pb, tbp, m, Xp, m, -1.0, 1.0, batch, st);
-    TF_PROF_END(tf::PROF_GJ_UPDATE, st);
     hipLaunchKernelGGL(k_panel_store<T>, dim3(gcopy, batch), dim3(256), 0, st, m, k0, pb, (T* const*)Xp,
                        (const T*)Tb);
   }
@@ -656,6 +601,8 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   const int m = h->m;
   V* rw = (V*)h->rw;
   V* tb = (V*)h->tb;
+  TF_PROF_WORK(tf::PROF_SOLVE, (double)h->solve_bytes);
+  TF_PROF_BEGIN(tf::PROF_SOLVE, st);
   hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
   const dim3 blk(256);
   for (size_t l = 0; l < h->lev.size(); ++l) {
@@ -671,6 +618,7 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
     hipLaunchKernelGGL((k_bwd_x<T, V>), dim3((m + 3) / 4, L.ne), blk, 0, st, m, L.e, (T* const*)L.Ye, (const V*)tb,
                        x);
   }
+  TF_PROF_END(tf::PROF_SOLVE, st);
   TF_LAUNCH_CHECK("bcr solve");
   return 0;
 }
@@ -853,6 +801,19 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single) {
     o.nnofb = (int)zfb.size();
     o.zfb = pb.add_ptrs(zfb);
     offs.push_back(o);
+  }
+  // one solve reads every Y once, the dense L/U of the lines eliminated at
+  // levels >= 1, the multipliers of every kept line, and moves ~6 vectors
+  {
+    size_t blocks = 0;
+    for (size_t l = 0; l < h->plan.size(); ++l) {
+      const LevelPlan& L = h->plan[l];
+      blocks += L.e.size();
+      if (l > 0)
+        for (size_t b = 0; b < L.e.size(); ++b) blocks += (L.ea[b] >= 0) + (L.eb[b] >= 0);
+      for (size_t b = 0; b < L.k.size(); ++b) blocks += (L.k1[b] >= 0) + (L.k2[b] >= 0);
+    }
+    h->solve_bytes = blocks * bm + 6 * sizeof(double) * (size_t)n;
   }
   pb.align();
   TF_HIP(hipMalloc(&h->pool, pb.bytes.size() + 64));

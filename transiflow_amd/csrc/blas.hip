@@ -20,6 +20,7 @@ struct ProfState {
   hipEvent_t open[PROF_IDS] = {};
   long count[PROF_IDS] = {};
   double ms[PROF_IDS] = {};
+  double work[PROF_IDS] = {};
 };
 ProfState& ps() {
   static ProfState s;
@@ -38,6 +39,12 @@ hipEvent_t take_event(ProfState& s) {
 }  // namespace
 
 bool prof_on() { return ps().on; }
+
+void prof_work(int id, double w) {
+  ProfState& s = ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  s.work[id] += w;
+}
 
 void prof_begin(int id, hipStream_t st) {
   ProfState& s = ps();
@@ -191,7 +198,18 @@ int tf_prof_reset(void) {
   for (int i = 0; i < tf::PROF_IDS; ++i) {
     s.count[i] = 0;
     s.ms[i] = 0.0;
+    s.work[i] = 0.0;
   }
+  return 0;
+}
+
+// Algorithmic work (flops or bytes, as the launch site states) summed over the
+// profiled launches of kernel id.
+int tf_prof_work(int id, double* work_total) {
+  if (id < 0 || id >= tf::PROF_IDS || !work_total) return tf::fail("bad profiling id");
+  tf::ProfState& s = tf::ps();
+  std::lock_guard<std::mutex> g(s.mu);
+  *work_total = s.work[id];
   return 0;
 }
 

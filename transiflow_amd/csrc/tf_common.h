@@ -41,11 +41,13 @@ enum ProfId : int {
   PROF_GEMV = 6,
   PROF_SPMV = 7,
   PROF_CONV = 8,
+  PROF_SOLVE = 9,
   PROF_IDS = 16
 };
 bool prof_on();
 void prof_begin(int id, hipStream_t st);
 void prof_end(int id, hipStream_t st);
+void prof_work(int id, double w);
 
 }  // namespace tf
 
@@ -56,6 +58,10 @@ void prof_end(int id, hipStream_t st);
 #define TF_PROF_END(id, st) \
   do {                      \
     if (tf::prof_on()) tf::prof_end((id), (st)); \
+  } while (0)
+#define TF_PROF_WORK(id, w) \
+  do {                      \
+    if (tf::prof_on()) tf::prof_work((id), (w)); \
   } while (0)
 
 #define TF_HIP(call)                                      \
