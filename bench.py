@@ -9,11 +9,19 @@ One "step" is one accepted pseudo-arclength step; the timed region is the
 whole continuation including the initial Newton solve and tangent, with the
 state resident in HBM.
 
-Variants: ``fp64`` (GMRES to 1e-10 with the exact FP64 line factorisation,
-refactored every Newton iteration) and ``mixed`` (FP32-stored factorisation
-reused across Newton iterations until GMRES needs more than a few iterations;
-GMRES, residuals and Newton in FP64).  ``value`` is the fastest variant that
-reproduces the FP64 variant's final state to 1e-8.
+Variants: ``fp64`` (FGMRES to a 1e-10 relative residual with the exact FP64
+line factorisation, refactored every Newton iteration) and ``mixed`` (FP64
+iterative refinement -- residual, update and convergence test in FP64 --
+around FP32 inner FGMRES with an FP32 line factorisation that is reused across
+Newton iterations and steps while the inner solves converge quickly).
+``value`` is the fastest variant that reproduces the FP64 variant's final
+state to 1e-8.
+
+The roofline object describes the kernel with the largest summed time in a
+profiled repeat of the headline variant (HIP events around each launch of the
+timed kernels, on the launch stream); its ``traffic`` comes from rocprofv3
+FETCH_SIZE / WRITE_SIZE counters committed in profiles/traffic.json, when that
+file names the kernel.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -31,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'continuation steps/s at 1e-8 residual: learned+mixed vs FP64 GPU; assembly GB/s'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -99,48 +108,61 @@ def assembly_roofline(tf, torch, itf, x, reps=20):
     }
 
 
-def solver_kernels(tf, torch, itf, x):
-    """Event-timed kernels of one factorisation + solve (line preconditioner)."""
+# profiled kernels: id -> (kernel name(s) as rocprofv3 lists them, bound, unit of the work figure)
+PROFILED = {
+    5: ('k_gemm_mfma', 'mfma', 'flop'),
+    3: ('k_panel', 'mfma', 'flop'),
+    9: ('line-factor apply (k_fwd, k_bwd_t, k_bwd_x, k_copy)', 'hbm', 'byte'),
+}
+
+
+def profiled_run(tf, torch, n, precision, target, ds, maxit):
+    """Repeat a timed continuation with the HIP-event profiler on; returns
+    per-kernel (launches, ms, work)."""
     from transiflow_amd import _lib
     lib = _lib.load()
     lib.tf_prof_reset()
     lib.tf_prof_enable(1)
-    J = itf.jacobian(x)
-    F = itf.rhs(x)
-    J.prec = None
-    itf.solver._lagged = None
-    y = itf.solve(J, F)
+    r = run_continuation(tf, torch, n, precision, target, ds, maxit)
     torch.cuda.synchronize()
     lib.tf_prof_enable(0)
-    del y
-    return kernel_times(lib, {'gj_panel': 3, 'gj_update': 4, 'gemm': 5, 'gemv': 6, 'spmv': 7}), itf.solver.line_stats()
+    out = {}
+    for pid in PROFILED:
+        c = ctypes.c_long()
+        ms = ctypes.c_double()
+        w = ctypes.c_double()
+        lib.tf_prof_read(pid, ctypes.byref(c), ctypes.byref(ms))
+        lib.tf_prof_work(pid, ctypes.byref(w))
+        out[pid] = (c.value, ms.value, w.value)
+    return r, out
 
 
-def cpu_baseline(n, x_host, mu, iters_total, accepted):
-    """Time the oracle (numpy/scipy restatement of the SciPy backend) for one
-    Newton-corrector iteration at the final state and scale by the GPU run's
-    iteration count (same algorithm, same counts)."""
+def cpu_baseline(n, x_host, mu, iters_total, accepted, reps=3):
+    """Time the oracle (numpy/scipy restatement of the SciPy backend) for
+    ``reps`` Newton-corrector iterations at the final state and scale the mean
+    by the GPU run's iteration count (same algorithm, same counts)."""
     import numpy
     from oracle.interface import Interface as OracleInterface
     ref = OracleInterface({'Reynolds Number': mu}, n, n)
     x = numpy.asarray(x_host)
     t0 = time.perf_counter()
-    ref.set_parameter('Reynolds Number', mu)
-    F = ref.rhs(x)
-    ref.set_parameter('Reynolds Number', mu + 1.0)
-    dF = ref.rhs(x) - F
-    ref.set_parameter('Reynolds Number', mu)
-    J = ref.jacobian(x)
-    ref.solve(J, -F)
-    ref.solve(J, dF)
-    t_iter = time.perf_counter() - t0
+    for _ in range(reps):
+        ref.set_parameter('Reynolds Number', mu)
+        F = ref.rhs(x)
+        ref.set_parameter('Reynolds Number', mu + 1.0)
+        dF = ref.rhs(x) - F
+        ref.set_parameter('Reynolds Number', mu)
+        J = ref.jacobian(x)
+        ref.solve(J, -F)
+        ref.solve(J, dF)
+    t_iter = (time.perf_counter() - t0) / reps
     est = t_iter * iters_total
     return {
         'value': accepted / est, 'unit': 'steps/s', 'cores': 1, 'kind': 'port',
-        'sample': ('one Newton-corrector iteration (2 rhs, 1 Jacobian, sparse LU via scipy splu, 2 solves) '
-                   'of the oracle at %dx%d, Re=%.0f, took %.2f s on %s (os.cpu_count()=%s, SuperLU single-threaded); '
+        'sample': ('%d Newton-corrector iterations (each 2 rhs, 1 Jacobian, sparse LU via scipy splu, 2 solves) '
+                   'of the oracle at %dx%d, Re=%.0f, %.2f s each on %s (os.cpu_count()=%s, SuperLU single-threaded); '
                    'scaled by the %d corrector/Newton iterations the same algorithm needed for %d accepted steps'
-                   % (n, n, mu, t_iter, platform.processor() or platform.machine(), os.cpu_count(),
+                   % (reps, n, n, mu, t_iter, platform.processor() or platform.machine(), os.cpu_count(),
                       iters_total, accepted)),
         'seconds_per_iteration': t_iter,
     }
@@ -169,8 +191,10 @@ def main():
 
     maxit = args.steps if args.steps > 0 else 1000
     if args.warmup > 0:
-        log('warm-up: %d step(s) at %dx%d' % (args.warmup, args.n, args.n))
-        run_continuation(tf, torch, args.n, 'fp64', args.target, args.ds, args.warmup)
+        for var in args.variants.split(','):
+            # also fills the GEMM tile tuner for this precision's shapes
+            log('warm-up (%s): %d step(s) at %dx%d' % (var, args.warmup, args.n, args.n))
+            run_continuation(tf, torch, args.n, var, args.target, args.ds, args.warmup)
 
     results = {}
     for var in args.variants.split(','):
@@ -193,32 +217,34 @@ def main():
 
     log('kernel timing sample')
     asm = assembly_roofline(tf, torch, hr['itf'], hr['x'])
-    kt, line_stats = solver_kernels(tf, torch, base['itf'] if base else hr['itf'], hr['x'])
-    # dominant kernel of the solve path
-    dom_name, (dom_calls, dom_ms) = max(kt.items(), key=lambda kv: kv[1][1])
-    m = line_stats['m']
-    if dom_name == 'gj_panel':
-        flops = 2.0 * 64 ** 3
-        roof = {'kernel': 'k_gj_panel (64x64 Gauss-Jordan panel inverse)', 'bound': 'mfma',
-                'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
-    elif dom_name == 'gj_update':
-        flops = 2.0 * 2 * 64 * m * m
-        roof = {'kernel': 'k_gj_update (rank-64 Gauss-Jordan update, FP64)', 'bound': 'mfma',
-                'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
-    elif dom_name == 'gemm':
-        flops = 2.0 * m ** 3
-        roof = {'kernel': 'dense FP64 block GEMM', 'bound': 'mfma', 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s'}
+    log('profiled repeat: %s' % head)
+    pr, kt = profiled_run(tf, torch, args.n, head, args.target, args.ds, maxit)
+    pid, (calls, ms, work) = max(kt.items(), key=lambda kv: kv[1][1])
+    name, bound, wunit = PROFILED[pid]
+    single = head == 'mixed'
+    if bound == 'mfma':
+        peak = FP32_PEAK_TFLOPS if single else FP64_PEAK_TFLOPS
+        achieved = work / (ms * 1e-3) / 1e12
+        unit = 'TFLOP/s'
     else:
-        flops = None
-        roof = {'kernel': dom_name, 'bound': 'hbm', 'peak': HBM_PEAK_GBS, 'unit': 'GB/s'}
-    avg_s = dom_ms / max(dom_calls, 1) * 1e-3
-    if flops is not None:
-        achieved = flops / avg_s / 1e12
-    else:
-        achieved = 8.0 * m * m / avg_s / 1e9
-    roof.update({'achieved': achieved, 'frac': achieved / roof['peak'], 'traffic': None,
-                 'avg_launch_us': avg_s * 1e6, 'launches': dom_calls,
-                 'per_launch_work': flops if flops is not None else 8.0 * m * m})
+        peak = HBM_PEAK_GBS
+        achieved = work / (ms * 1e-3) / 1e9
+        unit = 'GB/s'
+    traffic = None
+    tpath = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        key = '%s %s' % (name, 'f32' if single else 'f64')
+        if key in tj:
+            traffic = tj[key].get('bytes_per_launch')
+    roof = {'bound': bound, 'achieved': achieved, 'peak': peak, 'unit': unit, 'frac': achieved / peak,
+            'traffic': traffic, 'kernel': '%s (%s)' % (name, 'FP32' if single else 'FP64'),
+            'launches': calls, 'avg_launch_us': ms / max(calls, 1) * 1e3,
+            'per_launch_work': work / max(calls, 1), 'work_unit': wunit,
+            'profiled_run_steps_per_s': pr['steps_per_s'],
+            'all_profiled_kernels': {PROFILED[k][0]: {'launches': c, 'total_ms': t, 'work': w}
+                                     for k, (c, t, w) in kt.items()}}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -237,7 +263,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64' if head == 'fp64' else 'f64 state / f32-stored preconditioner',
+        'dtype': 'f64' if head == 'fp64' else 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
         'data': 'synthetic',
         'config': {
             'workload': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength '
@@ -253,7 +279,6 @@ def main():
         'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},
         'assembly': asm,
         'roofline': roof,
-        'solver_kernels_ms': {k: {'launches': c, 'total_ms': t} for k, (c, t) in kt.items()},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

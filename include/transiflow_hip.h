@@ -97,6 +97,10 @@ int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, con
                 float* y, int gauge, void* stream);
 int tf_set_entry_f64(double* x, long i, double v, void* stream);
 int tf_fill_f64(long n, double v, double* x, void* stream);
+/* FP32 vectors of the mixed-precision inner solve (no reference counterpart). */
+int tf_fill_f32(long n, double v, float* x, void* stream);
+int tf_axpby_f32(long n, double alpha, const float* x, double beta, const float* y, float* out, void* stream);
+int tf_scale_f32(long n, double alpha, const float* x, float* out, void* stream);
 
 /* ---- Krylov building blocks (scipy.sparse.linalg.gmres in SciPy.py:343-370) ----
  * V is a row-major basis with leading dimension ldv; h/coef are device FP64. */
@@ -140,6 +144,14 @@ int tf_bcr_destroy(tf_bcr* h);
 int tf_bcr_factor(tf_bcr* h, const int* row_ptr, const int* col, const double* val, int gauge, void* stream);
 int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
+int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
+
+/* ---- tile autotuner of the batched MFMA GEMM (no reference counterpart) ----
+ * mode 1 (default): the first GEMM of each (type, M, N, K, batch) times the tile
+ * candidates and keeps the fastest; mode 0: fixed rule.  tf_gemm_table copies rows
+ * (single, M, N, K, batch, tile) and returns the row count. */
+int tf_gemm_tuning(int mode);
+int tf_gemm_table(int* out_host, int max_rows);
 
 /* ---- opt-in kernel timing with HIP events (no reference counterpart; used by bench.py) ---- */
 int tf_prof_enable(int on);

@@ -19,13 +19,13 @@ GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
 pytestmark = pytest.mark.gpu
 
 
-def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64'):
+def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64', amplitude=1.0):
     from oracle.interface import Interface as OracleInterface
     p = dict(params)
     p['HIP Solver'] = {'Preconditioner': prec, 'Precision': precision}
     itf = hip.Interface(p, *shape, dim, dof)
     ref = OracleInterface(dict(params), *shape, dim, dof)
-    state = seeded_state(itf.n, seed)
+    state = seeded_state(itf.n, seed) * amplitude
     rhs = seeded_state(itf.n, seed + 100)
     J = itf.jacobian(state)
     y = itf.array_from_vector(itf.solve(J, rhs))
@@ -76,9 +76,23 @@ def test_solve_full_size_ldc256(hip):
     _solve_case(hip, {'Reynolds Number': 2000}, (256, 256, 1), 2, 3, 6)
 
 
-def test_solve_mixed_storage(hip):
-    """FP32-stored factor, FP64 GMRES: same 1e-10 residual bar."""
-    _solve_case(hip, {'Reynolds Number': 1000}, (64, 64, 1), 2, 3, 7, precision='mixed')
+@pytest.mark.parametrize('shape,re', [((64, 64, 1), 1000), ((128, 96, 1), 2000), ((256, 256, 1), 2000)])
+def test_solve_mixed_refinement(hip, shape, re):
+    """FP64 refinement around FP32 FGMRES with the FP32 line factor, at a state of
+    flow-like magnitude (|u| <= 0.1): FP64 residual to 1e-10 and the splu solution
+    to 1e-8, reached by the inner FP32 solves alone (>= 2 refinement steps, no
+    FP64 fallback)."""
+    itf = _solve_case(hip, {'Reynolds Number': re}, shape, 2, 3, 12, precision='mixed', amplitude=0.1)
+    assert itf.solver.refinements[-1] >= 2
+    assert itf.solver.fallbacks == 0
+
+
+def test_solve_mixed_fallback(hip):
+    """At a state of unit magnitude the line blocks reach condition numbers ~1e7 and
+    the FP32 factor cannot refine: the solve finishes in FP64 and still meets the
+    1e-10 bar."""
+    itf = _solve_case(hip, {'Reynolds Number': 2000}, (128, 96, 1), 2, 3, 12, precision='mixed')
+    assert itf.solver.fallbacks == 1
 
 
 def test_newton_ldc2d_16(hip):

@@ -63,11 +63,17 @@ SIGNATURES = {
     'tf_linelu_status': (_c_int, [_vp, _ip, _vp]),
     'tf_linelu_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
     'tf_linelu_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_gemm_tuning': (_c_int, [_c_int]),
+    'tf_gemm_table': (_c_int, [_ip, _c_int]),
     'tf_bcr_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
     'tf_bcr_destroy': (_c_int, [_vp]),
     'tf_bcr_factor': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),
     'tf_bcr_status': (_c_int, [_vp, _ip, _vp]),
     'tf_bcr_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_bcr_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_fill_f32': (_c_int, [_c_long, _c_double, _vp, _vp]),
+    'tf_axpby_f32': (_c_int, [_c_long, _c_double, _vp, _c_double, _vp, _vp, _vp]),
+    'tf_scale_f32': (_c_int, [_c_long, _c_double, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -40,6 +40,7 @@ SOURCES = {
     'krylov.hip': [],
     'linelu.hip': [],
     'bcr.hip': [],
+    'gemm.hip': [],
 }
 
 

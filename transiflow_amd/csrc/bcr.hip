@@ -494,14 +494,7 @@ void gemm(int M, int N, int K, void** A, int lda, void** B, int ldb, void** C, i
   if (batch <= 0) return;
   TF_PROF_WORK(tf::PROF_GEMM, 2.0 * M * N * K * batch);
   TF_PROF_BEGIN(tf::PROF_GEMM, st);
-  // 128 x 128 tiles when they alone fill the chip, 64 x 64 otherwise
-  const long big = (long)((N + 127) / 128) * ((M + 127) / 128) * batch;
-  if (big >= 512)
-    hipLaunchKernelGGL((tf::k_gemm_mfma<T, 128, 128>), dim3((N + 127) / 128, (M + 127) / 128, batch), dim3(256), 0,
-                       st, M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
-  else
-    hipLaunchKernelGGL((tf::k_gemm_mfma<T, 64, 64>), dim3((N + 63) / 64, (M + 63) / 64, batch), dim3(256), 0, st,
-                       M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta);
+  tf::gemm_batched<T>(M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta, batch, st);
   TF_PROF_END(tf::PROF_GEMM, st);
 }
 
@@ -902,6 +895,12 @@ int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream) {
   if (!h) return tf::fail("null handle");
   hipStream_t st = (hipStream_t)stream;
   return h->single ? solve_impl<float, double>(h, r, x, st) : solve_impl<double, double>(h, r, x, st);
+}
+
+int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream) {
+  if (!h) return tf::fail("null handle");
+  hipStream_t st = (hipStream_t)stream;
+  return h->single ? solve_impl<float, float>(h, r, x, st) : solve_impl<double, float>(h, r, x, st);
 }
 
 }  // extern "C"

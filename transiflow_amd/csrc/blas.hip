@@ -151,7 +151,8 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
 
 __global__ void k_set_entry(double* x, long i, double v) { x[i] = v; }
 
-__global__ __launch_bounds__(256) void k_fill(long n, double v, double* __restrict__ x) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(long n, double v, T* __restrict__ x) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
 }
 
@@ -220,7 +221,7 @@ int tf_set_entry_f64(double* x, long i, double v, void* stream) {
 }
 
 int tf_fill_f64(long n, double v, double* x, void* stream) {
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, v, x);
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, v, x);
   TF_LAUNCH_CHECK("k_fill");
   return 0;
 }
@@ -228,6 +229,24 @@ int tf_fill_f64(long n, double v, double* x, void* stream) {
 int tf_axpby_f64(long n, double alpha, const double* x, double beta, const double* y, double* out, void* stream) {
   hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, beta, y, out);
   TF_LAUNCH_CHECK("k_axpby");
+  return 0;
+}
+
+int tf_fill_f32(long n, double v, float* x, void* stream) {
+  hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, v, x);
+  TF_LAUNCH_CHECK("k_fill");
+  return 0;
+}
+
+int tf_axpby_f32(long n, double alpha, const float* x, double beta, const float* y, float* out, void* stream) {
+  hipLaunchKernelGGL(k_axpby<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, beta, y, out);
+  TF_LAUNCH_CHECK("k_axpby");
+  return 0;
+}
+
+int tf_scale_f32(long n, double alpha, const float* x, float* out, void* stream) {
+  hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, alpha, x, out);
+  TF_LAUNCH_CHECK("k_scale");
   return 0;
 }
 

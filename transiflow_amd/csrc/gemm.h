@@ -43,8 +43,8 @@ struct Mfma<float> {
   }
 };
 
-template <typename T, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm_mfma(int M, int N, int K, T* const* __restrict__ Ap, int lda,
+template <typename T, int BM, int BN, int MINW = 1>
+__global__ __launch_bounds__(256, MINW) void k_gemm_mfma(int M, int N, int K, T* const* __restrict__ Ap, int lda,
                                                    T* const* __restrict__ Bp, int ldb, T* const* __restrict__ Cp,
                                                    int ldc, double alpha, double beta) {
   constexpr int BK = 16;
@@ -135,6 +135,48 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(int M, int N, int K, T* const
         }
       }
     }
+}
+
+}  // namespace tf
+
+// ---- launch + tile autotuner ------------------------------------------------
+//
+// Candidate tiles: 128x128 and 64x64 (both hinted to two waves per SIMD) and
+// 128x64.  The first call with a given (type, M, N, K, batch) times every
+// candidate on scratch operands of that shape (every batch entry aliases one
+// scratch matrix, so the scratch stays at one A, B and C), and the fastest is
+// cached for the life of the process.  With tuning off a fixed rule is used.
+namespace tf {
+
+enum GemmTile : int { GEMM_128x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_TILES = 3 };
+
+template <typename T>
+inline void gemm_launch(int tile, int M, int N, int K, T* const* A, int lda, T* const* B, int ldb, T* const* C,
+                        int ldc, double alpha, double beta, int batch, hipStream_t st) {
+  switch (tile) {
+    case GEMM_128x128:
+      hipLaunchKernelGGL((k_gemm_mfma<T, 128, 128, 2>), dim3((N + 127) / 128, (M + 127) / 128, batch), dim3(256), 0,
+                         st, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+      break;
+    case GEMM_128x64:
+      hipLaunchKernelGGL((k_gemm_mfma<T, 128, 64, 1>), dim3((N + 63) / 64, (M + 127) / 128, batch), dim3(256), 0, st,
+                         M, N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm_mfma<T, 64, 64, 2>), dim3((N + 63) / 64, (M + 63) / 64, batch), dim3(256), 0, st, M,
+                         N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+  }
+}
+
+int gemm_tuned_tile(int single, int M, int N, int K, int batch, hipStream_t st);
+
+// C_b = alpha A_b B_b + beta C_b for b < batch (device pointer lists).
+template <typename T>
+inline void gemm_batched(int M, int N, int K, T* const* A, int lda, T* const* B, int ldb, T* const* C, int ldc,
+                         double alpha, double beta, int batch, hipStream_t st) {
+  if (batch <= 0 || M <= 0 || N <= 0) return;
+  const int tile = gemm_tuned_tile(sizeof(T) == 4, M, N, K, batch, st);
+  gemm_launch<T>(tile, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, batch, st);
 }
 
 }  // namespace tf

@@ -14,6 +14,8 @@
 // SciPy.direct_solve (SciPy.py:204-258) when preconditioned by a direct factor.
 #include "tf_common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int MD_THREADS = 256;
@@ -24,21 +26,23 @@ template <typename T>
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, const T* __restrict__ V, long ldv,
                                                                  const T* __restrict__ w,
                                                                  double* __restrict__ partial) {
-  // partial[i * MD_BLOCKS + b] = sum over block b's slice of V[i] . w
-  __shared__ double sh[MD_THREADS];
+  // partial[i * MD_BLOCKS + b] = sum over block b's slice of V[i] . w, accumulated
+  // in the vector type (FP32 vectors: FP32 accumulation inside each block)
+  using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  __shared__ Acc sh[MD_THREADS];
   const int chunk = (n + MD_BLOCKS - 1) / MD_BLOCKS;
   const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
   for (int i = 0; i < k; ++i) {
     const T* v = V + (long)i * ldv;
-    double acc = 0.0;
-    for (int e = b0 + threadIdx.x; e < b1; e += MD_THREADS) acc += (double)v[e] * (double)w[e];
+    Acc acc = 0;
+    for (int e = b0 + threadIdx.x; e < b1; e += MD_THREADS) acc += (Acc)v[e] * (Acc)w[e];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = MD_THREADS / 2; s > 0; s >>= 1) {
       if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[i * MD_BLOCKS + blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) partial[i * MD_BLOCKS + blockIdx.x] = (double)sh[0];
     __syncthreads();
   }
 }

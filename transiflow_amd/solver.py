@@ -14,9 +14,17 @@ Options (``parameters['HIP Solver']``, all optional):
                        dense line blocks fit the memory budget), ``'thomas'``
                        (the same factorisation by the sequential block Thomas
                        recursion), ``'block-jacobi'`` (cell blocks);
-``'Precision'``        ``'fp64'`` (default) or ``'mixed'``: the preconditioner is
-                       stored in FP32 and reused across Newton iterations while
-                       GMRES converges quickly; GMRES and the residual stay FP64;
+``'Precision'``        ``'fp64'`` (default) or ``'mixed'``: FP64 iterative
+                       refinement (residual b - A x, update and convergence test in
+                       FP64) around an FP32 inner FGMRES (FP32 basis, SpMV with FP32
+                       values, FP32 preconditioner factor and apply); the FP32
+                       factor is reused across Jacobians while the inner solve
+                       converges quickly;
+``'Inner Tolerance'``  (mixed) relative residual of each inner solve (1e-3: FP32
+                       solves stall near 1e-5 on these Jacobians, so the FP64
+                       refinement does the rest);
+``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
+``'Maximum Refinements'`` (mixed) outer refinement steps (20);
 ``'Tolerance'``        relative residual target (1e-10);
 ``'Restart'``          Krylov basis size (30);
 ``'Maximum Iterations'`` (300);
@@ -52,6 +60,9 @@ def _opts(parameters):
         'Restart': int(o.get('Restart', 30)),
         'Maximum Iterations': int(o.get('Maximum Iterations', 300)),
         'Reuse Limit': int(o.get('Reuse Limit', 8)),
+        'Inner Tolerance': float(o.get('Inner Tolerance', 1e-3)),
+        'Inner Iterations': int(o.get('Inner Iterations', 10)),
+        'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
     }
 
 
@@ -69,6 +80,10 @@ class CellBlockJacobi:
 
     def apply(self, r, out):
         _lib.call('tf_cellblock_apply_f64', self.ncell, self.dof, ptr(self.inv), ptr(r), ptr(out), stream())
+        return out
+
+    def apply32(self, r, out):
+        _lib.call('tf_cellblock_apply_f32', self.ncell, self.dof, ptr(self.inv), ptr(r), ptr(out), stream())
         return out
 
 
@@ -102,6 +117,10 @@ class LineFactor:
         _lib.call(self._p + '_solve_f64', self._h, ptr(r), ptr(out), stream())
         return out
 
+    def apply32(self, r, out):
+        _lib.call(self._p + '_solve_f32', self._h, ptr(r), ptr(out), stream())
+        return out
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and h.value:
@@ -128,47 +147,74 @@ def line_fits(itf, single=False):
 
 
 class Krylov:
-    """FP64 restarted flexible GMRES with CGS2 on device-resident vectors."""
+    """Restarted flexible GMRES (right preconditioning, CGS2 orthogonalisation)
+    on device-resident vectors of one precision: FP64, or FP32 for the inner
+    solve of the mixed-precision refinement (basis, SpMV, preconditioner apply
+    and block-level reductions in FP32).  The (j+2)-entry Hessenberg column is
+    reduced across blocks in FP64 and the Givens rotations run on the host."""
 
-    def __init__(self, n, restart):
+    _K = {
+        'float64': ('tf_axpby_f64', 'tf_scale_f64', 'tf_fill_f64', 'tf_multidot_f64', 'tf_multiaxpy_f64',
+                    'tf_normalize_f64'),
+        'float32': ('tf_axpby_f32', 'tf_scale_f32', 'tf_fill_f32', 'tf_multidot_f32', 'tf_multiaxpy_f32',
+                    'tf_normalize_f32'),
+    }
+
+    def __init__(self, n, restart, dtype='float64'):
         T = torch()
         self.n = n
         self.m = restart
-        self.V = T.empty((restart + 1, n), dtype=T.float64, device='cuda')
-        self.Z = T.empty((restart, n), dtype=T.float64, device='cuda')
-        self.w = T.empty(n, dtype=T.float64, device='cuda')
+        self.dtype = dtype
+        dt = getattr(T, dtype)
+        self.V = T.empty((restart + 1, n), dtype=dt, device='cuda')
+        self.Z = T.empty((restart, n), dtype=dt, device='cuda')
+        self.w = T.empty(n, dtype=dt, device='cuda')
         self.h = T.zeros(restart + 2, dtype=T.float64, device='cuda')
         self.y = T.empty(restart + 1, dtype=T.float64, device='cuda')
         nw = _lib.load().tf_krylov_work_size()
         self.work = T.empty(nw, dtype=T.float64, device='cuda')
         self.h_host = T.empty(restart + 2, dtype=T.float64).pin_memory()
+        (self._axpby, self._scale, self._fill, self._multidot, self._multiaxpy,
+         self._normalize) = self._K[dtype]
 
-    def solve(self, matvec, precond, b, x, tol, maxit):
+    def norm(self, x):
+        if self.dtype == 'float64':
+            return _norm(x, self.work)
+        _lib.call('tf_normalize_f32', x.numel(), ptr(x), ptr(self.work), ptr(self.h), ptr(self.w), stream())
+        self.h_host[:1].copy_(self.h[:1], non_blocking=True)
+        torch().cuda.current_stream().synchronize()
+        return float(self.h_host[0])
+
+    def solve(self, matvec, precond, b, x, tol, maxit, x_is_zero=False):
         """Solve A x = b (x holds the initial guess, updated in place).
         Returns (iterations, relative residual)."""
         n, m = self.n, self.m
         st = stream()
         V, Z, w, h = self.V, self.Z, self.w, self.h
         lib = _lib
-        bnrm = _norm(b, self.work)
+        bnrm = self.norm(b)
         if bnrm == 0.0:
-            lib.call('tf_fill_f64', n, 0.0, ptr(x), st)
+            lib.call(self._fill, n, 0.0, ptr(x), st)
             return 0, 0.0
         its = 0
         rel = math.inf
         first = True
         while True:
             # r = b - A x into V[0]
-            matvec(x, w)
-            lib.call('tf_axpby_f64', n, 1.0, ptr(b), -1.0, ptr(w), ptr(V[0]), st)
-            beta = _norm(V[0], self.work)
+            if first and x_is_zero:
+                V[0].copy_(b)
+                beta = bnrm
+            else:
+                matvec(x, w)
+                lib.call(self._axpby, n, 1.0, ptr(b), -1.0, ptr(w), ptr(V[0]), st)
+                beta = self.norm(V[0])
             rel = beta / bnrm
             if rel <= tol or its >= maxit:
                 return its, rel
             if not first and beta == 0.0:
                 return its, rel
             first = False
-            lib.call('tf_scale_f64', n, 1.0 / beta, ptr(V[0]), ptr(V[0]), st)
+            lib.call(self._scale, n, 1.0 / beta, ptr(V[0]), ptr(V[0]), st)
             H = numpy.zeros((m + 1, m))
             cs = numpy.zeros(m)
             sn = numpy.zeros(m)
@@ -178,12 +224,12 @@ class Krylov:
             for j in range(m):
                 precond(V[j], Z[j])
                 matvec(Z[j], w)
-                lib.call('tf_multidot_f64', n, j + 1, ptr(V), n, ptr(w), ptr(self.work), ptr(h), 0, st)
-                lib.call('tf_multiaxpy_f64', n, j + 1, ptr(V), n, ptr(h), ptr(w), -1.0, st)
-                lib.call('tf_multidot_f64', n, j + 1, ptr(V), n, ptr(w), ptr(self.work), ptr(self.y), 0, st)
-                lib.call('tf_multiaxpy_f64', n, j + 1, ptr(V), n, ptr(self.y), ptr(w), -1.0, st)
+                lib.call(self._multidot, n, j + 1, ptr(V), n, ptr(w), ptr(self.work), ptr(h), 0, st)
+                lib.call(self._multiaxpy, n, j + 1, ptr(V), n, ptr(h), ptr(w), -1.0, st)
+                lib.call(self._multidot, n, j + 1, ptr(V), n, ptr(w), ptr(self.work), ptr(self.y), 0, st)
+                lib.call(self._multiaxpy, n, j + 1, ptr(V), n, ptr(self.y), ptr(w), -1.0, st)
                 lib.call('tf_axpby_f64', j + 1, 1.0, ptr(h), 1.0, ptr(self.y), ptr(h), st)
-                lib.call('tf_normalize_f64', n, ptr(w), ptr(self.work), ctypes.c_void_p(h.data_ptr() + 8 * (j + 1)),
+                lib.call(self._normalize, n, ptr(w), ptr(self.work), ctypes.c_void_p(h.data_ptr() + 8 * (j + 1)),
                          ptr(V[j + 1]), st)
                 self.h_host[:j + 2].copy_(h[:j + 2], non_blocking=True)
                 torch().cuda.current_stream().synchronize()
@@ -211,7 +257,10 @@ class Krylov:
                 return its, rel
             yk = _back_substitute(H[:k, :k], g[:k])
             self.y[:k].copy_(torch().from_numpy(yk))
-            lib.call('tf_multiaxpy_f64', n, k, ptr(Z), n, ptr(self.y), ptr(x), 1.0, st)
+            lib.call(self._multiaxpy, n, k, ptr(Z), n, ptr(self.y), ptr(x), 1.0, st)
+            if its >= maxit and self.dtype == 'float32':
+                # inner solve of the refinement: the caller forms the true FP64 residual
+                return its, abs(g[k]) / bnrm
 
 
 def _back_substitute(R, g):
@@ -233,13 +282,18 @@ class Solver:
     def __init__(self, itf):
         self.itf = itf
         self.log = []
-        self._krylov = None
+        self.refinements = []  # (mixed) outer refinement steps per solve
+        self.fallbacks = 0     # (mixed) solves finished in FP64
+        self.trace = []        # (mixed) per refinement step: (inner its, inner estimate, outer FP64 rel)
+        self._krylov = {}
         self._lagged = None   # (precond) reused across Jacobians in mixed precision
+        self._buf = None
 
-    def _krylov_for(self, restart):
-        if self._krylov is None or self._krylov.m != restart:
-            self._krylov = Krylov(self.itf.n, restart)
-        return self._krylov
+    def _krylov_for(self, restart, dtype='float64'):
+        k = self._krylov.get(dtype)
+        if k is None or k.m != restart:
+            k = self._krylov[dtype] = Krylov(self.itf.n, restart, dtype)
+        return k
 
     def gauge(self):
         itf = self.itf
@@ -261,9 +315,11 @@ class Solver:
             return jac.prec, False
         if mixed and self._lagged is not None:
             jac.prec = self._lagged
+            jac.prec_exact = False   # factor of an earlier Jacobian
             return jac.prec, False
         t0 = time.perf_counter()
         jac.prec = self._build(jac, opts['Preconditioner'], single=mixed)
+        jac.prec_exact = True
         jac.prec_time = time.perf_counter() - t0
         if mixed:
             self._lagged = jac.prec
@@ -272,8 +328,7 @@ class Solver:
     def solve(self, jac, rhs, rhs2=None, V=None, W=None, C=None):
         if rhs2 is not None or V is not None:
             return self.solve_bordered(jac, rhs, rhs2, V, W, C)
-        itf = self.itf
-        opts = _opts(itf.parameters)
+        opts = _opts(self.itf.parameters)
         T = torch()
         b_vec = rhs if isinstance(rhs, DeviceVector) else DeviceVector.from_array(rhs)
         g = self.gauge()
@@ -282,6 +337,18 @@ class Solver:
             _lib.call('tf_set_entry_f64', ptr(b), g, 0.0, stream())   # SciPy.py:211-216
         x = T.empty_like(b)
         _lib.call('tf_fill_f64', x.numel(), 0.0, ptr(x), stream())
+        if opts['Precision'] == 'mixed':
+            its, rel, prec, fresh = self._solve_mixed(jac, b, x, opts)
+        else:
+            its, rel, prec, fresh = self._solve_fp64(jac, b, x, opts)
+        self.log.append((its, rel, getattr(prec, 'name', '?'), fresh))
+        if not rel <= opts['Tolerance']:
+            raise SolverError('GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
+                              % (rel, its))
+        return DeviceVector(x)
+
+    def _solve_fp64(self, jac, b, x, opts):
+        g = self.gauge()
         prec, fresh = self.preconditioner(jac, opts)
         kry = self._krylov_for(opts['Restart'])
 
@@ -289,18 +356,88 @@ class Solver:
             jac.matvec(v, out=out, gauge=g)
 
         its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
-        if opts['Precision'] == 'mixed' and not fresh and (its > opts['Reuse Limit'] or rel > opts['Tolerance']):
-            # lagged factor too stale: refactor this Jacobian and finish from the current iterate
-            jac.prec = None
-            self._lagged = None
-            prec, fresh = self.preconditioner(jac, opts)
-            its2, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
-            its += its2
-        self.log.append((its, rel, getattr(prec, 'name', '?'), fresh))
-        if not rel <= opts['Tolerance']:
-            raise SolverError('GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
-                              % (rel, its))
-        return DeviceVector(x)
+        return its, rel, prec, fresh
+
+    def _fp64_fallback(self, jac, b, x, opts):
+        prec = getattr(jac, 'prec64', None)
+        if prec is None:
+            kind = opts['Preconditioner']
+            prec = jac.prec64 = self._build(jac, kind, single=False)
+        kry = self._krylov_for(opts['Restart'])
+        g = self.gauge()
+
+        def matvec(v, out):
+            jac.matvec(v, out=out, gauge=g)
+
+        its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+        return its, rel, prec, True
+
+    def _buffers(self, n):
+        T = torch()
+        if self._buf is None or self._buf[0].numel() != n:
+            self._buf = (T.empty(n, dtype=T.float64, device='cuda'), T.empty(n, dtype=T.float32, device='cuda'),
+                         T.empty(n, dtype=T.float32, device='cuda'), T.empty(n, dtype=T.float64, device='cuda'))
+        return self._buf
+
+    def _solve_mixed(self, jac, b, x, opts):
+        """FP64 iterative refinement: r = b - A x (FP64), A d = r by FP32 FGMRES
+        with the FP32 factor, x += d (FP64), until ||r|| <= tol ||b|| in FP64."""
+        n = self.itf.n
+        g = self.gauge()
+        st = stream()
+        r, r32, d32, d = self._buffers(n)
+        kry = self._krylov_for(opts['Restart'], 'float32')
+        work = self._krylov_for(opts['Restart']).work
+        val32 = jac.values32()
+        prec, fresh = self.preconditioner(jac, opts)
+
+        def matvec32(v, out):
+            _lib.call('tf_spmv_f32', n, ptr(jac.row_ptr), ptr(jac.col), ptr(val32), ptr(v), ptr(out), int(g), st)
+
+        bnrm = _norm(b, work)
+        if bnrm == 0.0:
+            return 0, 0.0, prec, fresh
+        total, outer, rel, prev = 0, 0, 1.0, math.inf
+        inner_max = min(opts['Restart'], opts['Inner Iterations'])
+        while outer < opts['Maximum Refinements'] and total < opts['Maximum Iterations']:
+            if outer > 0:
+                jac.matvec(x, out=r, gauge=g)
+                _lib.call('tf_axpby_f64', n, 1.0, ptr(b), -1.0, ptr(r), ptr(r), st)
+                rel = _norm(r, work) / bnrm
+                if rel <= opts['Tolerance']:
+                    break
+                if rel > 0.5 * prev and jac.prec_exact:
+                    # refinement stagnates with this Jacobian's own FP32 factor (FP32 too
+                    # coarse for its conditioning): finish in FP64 from the current x
+                    its64, rel, prec, _ = self._fp64_fallback(jac, b, x, opts)
+                    total += its64
+                    self.fallbacks += 1
+                    self.refinements.append(outer)
+                    return total, rel, prec, True
+                prev = rel
+                _lib.call('tf_f64_to_f32', n, ptr(r), ptr(r32), st)
+            else:
+                _lib.call('tf_f64_to_f32', n, ptr(b), ptr(r32), st)
+            _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
+            its, est = kry.solve(matvec32, prec.apply32, r32, d32, opts['Inner Tolerance'], inner_max,
+                                 x_is_zero=True)
+            self.trace.append((its, est, rel))
+            total += its
+            _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)
+            _lib.call('tf_axpby_f64', n, 1.0, ptr(x), 1.0, ptr(d), ptr(x), st)
+            outer += 1
+            if not jac.prec_exact and its > opts['Reuse Limit']:
+                # lagged factor too stale: refactor this Jacobian and keep refining
+                jac.prec = None
+                self._lagged = None
+                prec, fresh = self.preconditioner(jac, opts)
+                prev = math.inf
+        if outer >= opts['Maximum Refinements'] or total >= opts['Maximum Iterations']:
+            jac.matvec(x, out=r, gauge=g)
+            _lib.call('tf_axpby_f64', n, 1.0, ptr(b), -1.0, ptr(r), ptr(r), st)
+            rel = _norm(r, work) / bnrm
+        self.refinements.append(outer)
+        return total, rel, prec, fresh
 
     def line_stats(self):
         """Shape of the line factorisation for this problem (rows per line, lines)."""

@@ -209,6 +209,16 @@ class DeviceCsrMatrix:
         self.lu = None
         self.bordered_lu = False
         self.prec = None
+        self.prec_exact = False
+        self._val32 = None
+
+    def values32(self):
+        """FP32 copy of the values (cached; used by the mixed-precision inner solve)."""
+        if self._val32 is None:
+            T = torch()
+            self._val32 = T.empty(self.val.numel(), dtype=T.float32, device='cuda')
+            _lib.call('tf_f64_to_f32', self.val.numel(), ptr(self.val), ptr(self._val32), stream())
+        return self._val32
 
     @property
     def shape(self):
