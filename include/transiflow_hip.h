@@ -146,6 +146,23 @@ int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
 int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
 
+/* ---- learned preconditioner: convolutions on the cell grid (no reference counterpart) ----
+ * Activations cell-major [ncell][C] in BF16 (uint16 storage); layer weights in the
+ * padded MFMA layout [kpad][npad] (K = taps*cin rows, row tap*cin + ci, taps = 3^dim,
+ * tap = (dx+1) + 3(dy+1) [+ 9(dz+1) in 3D]); zero padding outside the grid; FP32 accumulation. */
+int tf_conv_fwd(int nx, int ny, int nz, int dim, int cin, int cout, int kpad, int npad, const void* in,
+                const void* W, const float* bias, int relu, float* out32, void* out16, void* stream);
+int tf_conv_wgrad_work_size(long ncell, int kpad, int npad);
+int tf_conv_wgrad(int nx, int ny, int nz, int dim, int cin, int cout, int kpad, int npad, const void* in,
+                  const void* gout16, const float* g32, float* work, float* dW, float* db, void* stream);
+int tf_mse_grad(long n, const float* y, const float* t, float* g, float* loss_partial, void* stream);
+int tf_relu_bwd(long n, const float* gin, const void* act16, int relu, float* g32, void* g16, void* stream);
+int tf_conv_pack(int dim, int cin, int cout, int kpad, int npad, int kpad_t, int npad_t, const float* W, void* Wf,
+                 void* Wt, void* stream);
+int tf_adam(long n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float eps, int step,
+            void* stream);
+int tf_to_bf16(long n, const float* x, float scale, void* y, void* stream);
+
 /* ---- tile autotuner of the batched MFMA GEMM (no reference counterpart) ----
  * mode 1 (default): the first GEMM of each (type, M, N, K, batch) times the tile
  * candidates and keeps the fastest; mode 0: fixed rule.  tf_gemm_table copies rows

@@ -63,6 +63,17 @@ SIGNATURES = {
     'tf_linelu_status': (_c_int, [_vp, _ip, _vp]),
     'tf_linelu_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
     'tf_linelu_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_conv_fwd': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp,
+                             _c_int, _vp, _vp, _vp]),
+    'tf_conv_wgrad_work_size': (_c_int, [_c_long, _c_int, _c_int]),
+    'tf_conv_wgrad': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                               _vp, _vp, _vp]),
+    'tf_mse_grad': (_c_int, [_c_long, _vp, _vp, _vp, _vp, _vp]),
+    'tf_relu_bwd': (_c_int, [_c_long, _vp, _vp, _c_int, _vp, _vp, _vp]),
+    'tf_conv_pack': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    'tf_adam': (_c_int, [_c_long, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, _c_int, _vp]),
+    'tf_to_bf16': (_c_int, [_c_long, _vp, ctypes.c_float, _vp, _vp]),
     'tf_gemm_tuning': (_c_int, [_c_int]),
     'tf_gemm_table': (_c_int, [_ip, _c_int]),
     'tf_bcr_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),

@@ -41,6 +41,7 @@ SOURCES = {
     'linelu.hip': [],
     'bcr.hip': [],
     'gemm.hip': [],
+    'conv.hip': [],
 }
 
 

@@ -24,6 +24,10 @@ Options (``parameters['HIP Solver']``, all optional):
                        solves stall near 1e-5 on these Jacobians, so the FP64
                        refinement does the rest);
 ``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
+``'Learned Network'``  (mixed) a ``learned.ConvNet`` (or the path of its saved
+                       weights): the inner preconditioner becomes the line
+                       factor followed by the judged network correction
+                       (``learned.LearnedCorrection``);
 ``'Maximum Refinements'`` (mixed) outer refinement steps (20);
 ``'Tolerance'``        relative residual target (1e-10);
 ``'Restart'``          Krylov basis size (30);
@@ -63,6 +67,7 @@ def _opts(parameters):
         'Inner Tolerance': float(o.get('Inner Tolerance', 1e-3)),
         'Inner Iterations': int(o.get('Inner Iterations', 10)),
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
+        'Learned Network': o.get('Learned Network'),
     }
 
 
@@ -285,6 +290,8 @@ class Solver:
         self.refinements = []  # (mixed) outer refinement steps per solve
         self.fallbacks = 0     # (mixed) solves finished in FP64
         self.trace = []        # (mixed) per refinement step: (inner its, inner estimate, outer FP64 rel)
+        self.observer = None   # optional callable(jac, precond32, v) on every inner preconditioner input
+        self._learned = None
         self._krylov = {}
         self._lagged = None   # (precond) reused across Jacobians in mixed precision
         self._buf = None
@@ -358,6 +365,15 @@ class Solver:
         its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
         return its, rel, prec, fresh
 
+    def _learned_for(self, net):
+        if self._learned is None:
+            from .learned import ConvNet, LearnedCorrection
+            if isinstance(net, str):
+                net = ConvNet.load(net)
+            itf = self.itf
+            self._learned = LearnedCorrection(net, (itf.nx, itf.ny, itf.nz), itf.n)
+        return self._learned
+
     def _fp64_fallback(self, jac, b, x, opts):
         prec = getattr(jac, 'prec64', None)
         if prec is None:
@@ -397,6 +413,24 @@ class Solver:
         bnrm = _norm(b, work)
         if bnrm == 0.0:
             return 0, 0.0, prec, fresh
+
+        def inner_prec():
+            p = prec
+            net = opts['Learned Network']
+            if net is not None:
+                p = self._learned_for(net).bind(p, matvec32)
+            if self.observer is not None:
+                obs, base = self.observer, p
+
+                class _Observed:
+                    name = getattr(base, 'name', '?')
+
+                    @staticmethod
+                    def apply32(v, out):
+                        obs(jac, base, v)
+                        return base.apply32(v, out)
+                p = _Observed
+            return p
         total, outer, rel, prev = 0, 0, 1.0, math.inf
         inner_max = min(opts['Restart'], opts['Inner Iterations'])
         while outer < opts['Maximum Refinements'] and total < opts['Maximum Iterations']:
@@ -419,7 +453,7 @@ class Solver:
             else:
                 _lib.call('tf_f64_to_f32', n, ptr(b), ptr(r32), st)
             _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
-            its, est = kry.solve(matvec32, prec.apply32, r32, d32, opts['Inner Tolerance'], inner_max,
+            its, est = kry.solve(matvec32, inner_prec().apply32, r32, d32, opts['Inner Tolerance'], inner_max,
                                  x_is_zero=True)
             self.trace.append((its, est, rel))
             total += its
