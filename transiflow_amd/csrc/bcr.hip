@@ -32,6 +32,7 @@
 #include "gemm.h"
 #include "tf_common.h"
 
+#include <cstdint>
 #include <vector>
 
 namespace {
@@ -314,29 +315,65 @@ __global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ 
 }
 
 // ---- solve ----------------------------------------------------------------------
+//
+// Every solve kernel is a batch of row-times-vector products over m-long rows.
+// A 16-lane group owns one row and reads it with 16-byte loads when the row
+// length and the vectors allow it (16 rows per 256-thread block).
 
-// rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines; one wave per row.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int W = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int W = 2;
+};
+
+// sum over c of a[c] x[c] by the 16-lane group g (the result is in every lane of the group)
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void k_fwd(int m, const int* __restrict__ k_line, const int* __restrict__ e1,
-                                             const int* __restrict__ e2, T* const* __restrict__ P1p,
-                                             T* const* __restrict__ P2p, V* __restrict__ rw) {
-  const int b = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= m) return;
+__device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g,
+                                            bool vec) {
   double acc = 0.0;
-  if (e1[b] >= 0) {
-    const T* p1 = P1p[b] + (long)r * m;
-    const V* x1 = rw + (long)e1[b] * m;
-    for (int c = lane; c < m; c += 64) acc += (double)p1[c] * (double)x1[c];
+  if constexpr (sizeof(T) == sizeof(V)) {
+    if (vec) {
+      typedef typename Vec16<T>::type VT;
+      constexpr int W = Vec16<T>::W;
+      const VT* av = (const VT*)a;
+      const VT* xv = (const VT*)x;
+      for (int c = g; c < m / W; c += 16) {
+        const VT p = av[c], q = xv[c];
+#pragma unroll
+        for (int w = 0; w < W; ++w) acc += (double)p[w] * (double)q[w];
+      }
+    } else {
+      for (int c = g; c < m; c += 16) acc += (double)a[c] * (double)x[c];
+    }
+  } else {
+    for (int c = g; c < m; c += 16) acc += (double)a[c] * (double)x[c];
   }
-  if (e2[b] >= 0) {
-    const T* p2 = P2p[b] + (long)r * m;
-    const V* x2 = rw + (long)e2[b] * m;
-    for (int c = lane; c < m; c += 64) acc += (double)p2[c] * (double)x2[c];
-  }
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) {
+  for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+  return acc;
+}
+
+// rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void k_fwd(int m, int vec, const int* __restrict__ k_line,
+                                             const int* __restrict__ e1, const int* __restrict__ e2,
+                                             T* const* __restrict__ P1p, T* const* __restrict__ P2p,
+                                             V* __restrict__ rw) {
+  const int b = blockIdx.y;
+  const int g = threadIdx.x & 15;
+  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const bool live = r < m;
+  const int rr = live ? r : 0;
+  double acc = 0.0;
+  if (e1[b] >= 0) acc += group_dot(P1p[b] + (long)rr * m, rw + (long)e1[b] * m, m, g, vec);
+  if (e2[b] >= 0) acc += group_dot(P2p[b] + (long)rr * m, rw + (long)e2[b] * m, m, g, vec);
+  if (live && g == 0) {
     V* out = rw + (long)k_line[b] * m + r;
     *out = (V)((double)*out - acc);
   }
@@ -344,7 +381,7 @@ __global__ __launch_bounds__(256) void k_fwd(int m, const int* __restrict__ k_li
 
 // t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, or the level-0 sparse rows)
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void k_bwd_t(int m, int sparse, const int* __restrict__ e_line,
+__global__ __launch_bounds__(256) void k_bwd_t(int m, int vec, int sparse, const int* __restrict__ e_line,
                                                const int* __restrict__ a_line, const int* __restrict__ b_line,
                                                T* const* __restrict__ Lp, T* const* __restrict__ Up,
                                                const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
@@ -352,47 +389,40 @@ __global__ __launch_bounds__(256) void k_bwd_t(int m, int sparse, const int* __r
                                                const V* __restrict__ rw, const V* __restrict__ x,
                                                V* __restrict__ tbuf) {
   const int bi = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= m) return;
-  const long row = (long)e_line[bi] * m + r;
-  double acc = 0.0;
+  const int g = threadIdx.x & 15;
+  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const bool live = r < m;
+  const int rr = live ? r : 0;
+  const long row = (long)e_line[bi] * m + rr;
   const int a = a_line[bi], bb = b_line[bi];
+  double acc = 0.0;
   if (sparse) {
-    if (lane == 0) {
+    if (live && g == 0) {
       if (a >= 0)
         for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)x[(long)a * m + Lr[row * KC + q].idx];
       if (bb >= 0)
         for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
     }
   } else {
-    if (a >= 0) {
-      const T* l = Lp[bi] + (long)r * m;
-      for (int c = lane; c < m; c += 64) acc += (double)l[c] * (double)x[(long)a * m + c];
-    }
-    if (bb >= 0) {
-      const T* u = Up[bi] + (long)r * m;
-      for (int c = lane; c < m; c += 64) acc += (double)u[c] * (double)x[(long)bb * m + c];
-    }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (a >= 0) acc += group_dot(Lp[bi] + (long)rr * m, x + (long)a * m, m, g, vec);
+    if (bb >= 0) acc += group_dot(Up[bi] + (long)rr * m, x + (long)bb * m, m, g, vec);
   }
-  if (lane == 0) tbuf[row] = (V)((double)rw[row] - acc);
+  if (live && g == 0) tbuf[row] = (V)((double)rw[row] - acc);
 }
 
 // x[e] = Y_e t[e]
 template <typename T, typename V>
-__global__ __launch_bounds__(256) void k_bwd_x(int m, const int* __restrict__ e_line, T* const* __restrict__ Yp,
-                                               const V* __restrict__ tbuf, V* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_bwd_x(int m, int vec, const int* __restrict__ e_line,
+                                               T* const* __restrict__ Yp, const V* __restrict__ tbuf,
+                                               V* __restrict__ x) {
   const int bi = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (r >= m) return;
+  const int g = threadIdx.x & 15;
+  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const bool live = r < m;
+  const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
-  const T* y = Yp[bi] + (long)r * m;
-  double acc = 0.0;
-  for (int c = lane; c < m; c += 64) acc += (double)y[c] * (double)tbuf[base + c];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) x[base + r] = (V)acc;
+  const double acc = group_dot(Yp[bi] + (long)rr * m, tbuf + base, m, g, vec);
+  if (live && g == 0) x[base + r] = (V)acc;
 }
 
 template <typename V>
@@ -598,17 +628,21 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   TF_PROF_BEGIN(tf::PROF_SOLVE, st);
   hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
   const dim3 blk(256);
+  // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
+  constexpr int W = 16 / (int)sizeof(T);
+  const int vec = (sizeof(T) == sizeof(V) && m % W == 0 && ((uintptr_t)x & 15) == 0) ? 1 : 0;
+  const unsigned gx = (unsigned)((m + 15) / 16);
   for (size_t l = 0; l < h->lev.size(); ++l) {
     BcrLevelDev& L = h->lev[l];
     if (L.nk)
-      hipLaunchKernelGGL((k_fwd<T, V>), dim3((m + 3) / 4, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, (T* const*)L.P1,
+      hipLaunchKernelGGL((k_fwd<T, V>), dim3(gx, L.nk), blk, 0, st, m, vec, L.k, L.k1, L.k2, (T* const*)L.P1,
                          (T* const*)L.P2, rw);
   }
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     BcrLevelDev& L = h->lev[l];
-    hipLaunchKernelGGL((k_bwd_t<T, V>), dim3((m + 3) / 4, L.ne), blk, 0, st, m, l == 0 ? 1 : 0, L.e, L.ea, L.eb,
+    hipLaunchKernelGGL((k_bwd_t<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, l == 0 ? 1 : 0, L.e, L.ea, L.eb,
                        (T* const*)L.Le, (T* const*)L.Ue, h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)x, tb);
-    hipLaunchKernelGGL((k_bwd_x<T, V>), dim3((m + 3) / 4, L.ne), blk, 0, st, m, L.e, (T* const*)L.Ye, (const V*)tb,
+    hipLaunchKernelGGL((k_bwd_x<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, (T* const*)L.Ye, (const V*)tb,
                        x);
   }
   TF_PROF_END(tf::PROF_SOLVE, st);
