@@ -168,6 +168,31 @@ def cpu_baseline(n, x_host, mu, iters_total, accepted, reps=3):
     }
 
 
+def kernel_sample(n, reps=10):
+    """Short launch sequence for hardware-counter runs: one FP32 line
+    factorisation of the n x n cavity Jacobian at Re = 1000, ``reps`` FP32
+    applies, and a plain copy of known size (out = x - y over 64 Mi floats:
+    512 MiB read, 256 MiB written) to calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    import transiflow_amd as tf
+    from transiflow_amd import _lib
+    from transiflow_amd.vector import ptr, stream
+    itf = tf.Interface({'Reynolds Number': 1000.0}, n, n)
+    J = itf.jacobian(itf.vector())
+    P = itf.solver._build(J, 'line', True)
+    r = torch.ones(itf.n, dtype=torch.float32, device='cuda')
+    z = torch.empty_like(r)
+    for _ in range(reps):
+        P.apply32(r, z)
+    big = 64 * 2 ** 20
+    a = torch.ones(big, dtype=torch.float32, device='cuda')
+    b = torch.ones(big, dtype=torch.float32, device='cuda')
+    c = torch.empty(big, dtype=torch.float32, device='cuda')
+    _lib.call('tf_axpby_f32', big, 1.0, ptr(a), -1.0, ptr(b), ptr(c), stream())
+    torch.cuda.synchronize()
+    log('kernel sample done: %d applies, %d-float copy' % (reps, big))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -179,11 +204,17 @@ def main():
     ap.add_argument('--ds', type=float, default=100.0)
     ap.add_argument('--variants', default='fp64,mixed')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--kernel-sample', action='store_true',
+                    help='run only the short launch sequence used for hardware-counter profiles')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the continuation hot path on a single MI355X '
               '(multi-GPU sharding is described in DESIGN.md, not implemented); use --gpus 1', file=sys.stderr)
         sys.exit(2)
+
+    if args.kernel_sample:
+        kernel_sample(args.n)
+        return
 
     import numpy
     import torch

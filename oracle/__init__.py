@@ -26,5 +26,7 @@ Pinning (what checks each function against something other than itself):
 * layout.py -- closed-form coordinate-vector tests and the index formula;
 * interface.py solve -- residual identities and scipy's dense solver on
   tiny systems; bordered matrix identity;
-* conv.py -- finite differences and a direct loop convolution on tiny inputs.
+* conv.py -- scipy.signal.correlate on single-channel layers, a brute-force
+  loop convolution on tiny inputs, finite differences of forward vs backward
+  and BF16 rounding against known bit patterns.
 """

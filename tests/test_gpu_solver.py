@@ -129,3 +129,52 @@ def test_continuation_ldc2d_8(hip):
     print('continuation 8x8: final mu %.6f, rel err of final state %.2e' % (mu, err))
     assert abs(mu - float(g['mu'])) <= 1e-6
     assert err <= 1e-8
+
+
+@pytest.mark.parametrize('k', [1, 2])
+def test_bordered_solve_matches_oracle(hip, k):
+    """[J~, sV; sW~^T, s^2 C] solve (SciPy.py:51-129,204-258) against the oracle's
+    sparse LU of the assembled bordered matrix, with k border columns."""
+    from oracle.interface import Interface as OracleInterface
+    params = {'Reynolds Number': 300}
+    itf = hip.Interface(dict(params), 24, 20)
+    ref = OracleInterface(dict(params), 24, 20)
+    n = itf.n
+    rng = numpy.random.default_rng(21)
+    state = rng.uniform(-1, 1, n) * 0.1
+    f = rng.uniform(-1, 1, n)
+    if k == 1:
+        V, W, C, r2 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), 0.7, 0.3
+    else:
+        V, W = rng.uniform(-1, 1, (n, k)), rng.uniform(-1, 1, (n, k))
+        C, r2 = rng.uniform(-1, 1, (k, k)) + 2 * numpy.eye(k), rng.uniform(-1, 1, k)
+    y1, y2 = itf.solve(itf.jacobian(state), f, r2, V, W, C)
+    z1, z2 = ref.solve(ref.jacobian(state), f, r2, V, W, C)
+    y1 = itf.array_from_vector(y1)
+    err1 = numpy.linalg.norm(y1 - z1) / numpy.linalg.norm(z1)
+    err2 = numpy.linalg.norm(numpy.atleast_1d(y2) - numpy.atleast_1d(z2)) / numpy.linalg.norm(numpy.atleast_1d(z2))
+    print('bordered k=%d: |y1 - ref| %.2e, |y2 - ref| %.2e' % (k, err1, err2))
+    assert err1 <= 1e-8 and err2 <= 1e-8
+    assert numpy.ndim(y2) == (0 if k == 1 else 1)
+
+
+def test_bordered_continuation_matches_oracle(hip):
+    """Continuation(bordered_solver=True) (Continuation.py:152-155): HIP backend vs
+    the same driver on the oracle backend, LDC 8x8, Re 0 -> 2000."""
+    from oracle.interface import Interface as OracleInterface
+    out = {}
+    for name, itf in (('hip', hip.Interface({}, 8, 8)), ('oracle', OracleInterface({}, 8, 8, 1, 2, 3))):
+        cont = hip.Continuation(itf, newton_tolerance=1e-8, bordered_solver=True, printer=None)
+        mus, its = [], []
+
+        def cb(_i, _x, mu, mus=mus, its=its, cont=cont):
+            mus.append(mu)
+            its.append(cont.newton_iterations)
+        x0 = cont.newton(itf.vector())
+        x, mu = cont.continuation(x0, 'Reynolds Number', 0, 2000, 100, callback=cb)
+        out[name] = (numpy.asarray(itf.array_from_vector(x) if name == 'hip' else x), mus, its, mu)
+    (xh, mh, ih, muh), (xo, mo, io, muo) = out['hip'], out['oracle']
+    assert ih == io
+    numpy.testing.assert_allclose(mh, mo, rtol=1e-8, atol=1e-8)
+    assert numpy.linalg.norm(xh - xo) / numpy.linalg.norm(xo) <= 1e-8
+    assert abs(muh - muo) <= 1e-8

@@ -335,13 +335,18 @@ class Solver:
     def solve(self, jac, rhs, rhs2=None, V=None, W=None, C=None):
         if rhs2 is not None or V is not None:
             return self.solve_bordered(jac, rhs, rhs2, V, W, C)
+        b_vec = rhs if isinstance(rhs, DeviceVector) else DeviceVector.from_array(rhs)
+        return DeviceVector(self._solve_device(jac, b_vec.data, zero_gauge=True))
+
+    def _solve_device(self, jac, rhs, zero_gauge):
+        """x with (gauged J) x = rhs; ``zero_gauge`` zeroes rhs[pressure_row] first
+        (SciPy.py:211-216; the bordered solve keeps it, SciPy.py:226-227)."""
         opts = _opts(self.itf.parameters)
         T = torch()
-        b_vec = rhs if isinstance(rhs, DeviceVector) else DeviceVector.from_array(rhs)
         g = self.gauge()
-        b = b_vec.data.clone()
-        if g >= 0:
-            _lib.call('tf_set_entry_f64', ptr(b), g, 0.0, stream())   # SciPy.py:211-216
+        b = rhs.clone()
+        if g >= 0 and zero_gauge:
+            _lib.call('tf_set_entry_f64', ptr(b), g, 0.0, stream())
         x = T.empty_like(b)
         _lib.call('tf_fill_f64', x.numel(), 0.0, ptr(x), stream())
         if opts['Precision'] == 'mixed':
@@ -352,7 +357,7 @@ class Solver:
         if not rel <= opts['Tolerance']:
             raise SolverError('GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
                               % (rel, its))
-        return DeviceVector(x)
+        return x
 
     def _solve_fp64(self, jac, b, x, opts):
         g = self.gauge()
@@ -479,4 +484,55 @@ class Solver:
         return {'m': m, 'lines': self.itf.n // m}
 
     def solve_bordered(self, jac, rhs, rhs2, V, W, C):
-        raise NotImplementedError('bordered solves are not available yet in the HIP backend')
+        """Bordered solve of SciPy.direct_solve (SciPy.py:51-129,204-258):
+
+            [J~      s V  ] [y1]   [rhs     ]
+            [s W~^T  s^2 C] [z ] = [s rhs2  ],   returns (y1, s z)
+
+        with s = border_scaling, J~ the gauged Jacobian, W~ = W with the gauge
+        entry dropped (the border row skips the gauge column) and the gauge
+        entry of rhs kept (the bordered right-hand side is not gauged).  Solved
+        by block elimination with the same factor: J~ a = rhs, J~ B = s V,
+        z = (s^2 C - s W~^T B)^-1 (s rhs2 - s W~^T a), y1 = a - B z."""
+        itf = self.itf
+        n = itf.n
+        s = itf.border_scaling
+        g = self.gauge()
+        T = torch()
+
+        def columns(M):
+            if M is None:
+                return None
+            if isinstance(M, DeviceVector):
+                return [M.data]
+            a = numpy.asarray(M, dtype=float)
+            if a.ndim == 1:
+                a = a[:, None]
+            return [T.from_numpy(numpy.ascontiguousarray(a[:, j])).cuda() for j in range(a.shape[1])]
+
+        Vc = columns(V)
+        Wc = columns(W) if W is not None else Vc
+        k = len(Vc)
+        scalar = numpy.ndim(rhs2) == 0
+        r2 = numpy.atleast_1d(numpy.asarray(rhs2, dtype=float))
+        Cm = numpy.zeros((k, k)) if C is None else numpy.atleast_2d(numpy.asarray(C, dtype=float))
+        b = rhs.data if isinstance(rhs, DeviceVector) else T.from_numpy(numpy.ascontiguousarray(rhs, dtype=float)).cuda()
+        a = self._solve_device(jac, b, zero_gauge=False)
+        Bc = []
+        for v in Vc:
+            sv = T.empty_like(v)
+            _lib.call('tf_scale_f64', n, s, ptr(v), ptr(sv), stream())
+            Bc.append(self._solve_device(jac, sv, zero_gauge=False))
+        Wt = []
+        for w in Wc:
+            wt = w.clone()
+            if g >= 0:
+                _lib.call('tf_set_entry_f64', ptr(wt), g, 0.0, stream())
+            Wt.append(DeviceVector(wt))
+        S = s * s * Cm - s * numpy.array([[Wt[i].dot(DeviceVector(Bc[j])) for j in range(k)] for i in range(k)])
+        rr = s * r2 - s * numpy.array([Wt[i].dot(DeviceVector(a)) for i in range(k)])
+        z = numpy.linalg.solve(S, rr)
+        for j in range(k):
+            _lib.call('tf_axpby_f64', n, 1.0, ptr(a), -float(z[j]), ptr(Bc[j]), ptr(a), stream())
+        y2 = z * s
+        return DeviceVector(a), (float(y2[0]) if scalar else y2)
