@@ -137,9 +137,11 @@ int tf_linelu_solve_f32(tf_linelu* h, const float* r, float* x, void* stream);
 
 /* ---- block cyclic reduction over grid lines (same role as tf_linelu, log-depth) ----
  * Odd-even reduction of the block-tridiagonal gauged Jacobian; every level is a batch of
- * dense inversions and GEMMs.  single != 0 factors and stores in FP32. */
+ * dense inversions and GEMMs.  single != 0 factors and stores in FP32.  ylines != 0 (2D)
+ * takes the grid columns as lines (m = ny*dof; nx and dof describe the grid) so that wide
+ * domains factor short lines; vectors stay in the natural order. */
 typedef struct tf_bcr tf_bcr;
-int tf_bcr_create(tf_bcr** out, int n, int m, int single);
+int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int ylines);
 int tf_bcr_destroy(tf_bcr* h);
 int tf_bcr_factor(tf_bcr* h, const int* row_ptr, const int* col, const double* val, int gauge, void* stream);
 int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);

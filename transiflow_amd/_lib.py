@@ -76,7 +76,7 @@ SIGNATURES = {
     'tf_to_bf16': (_c_int, [_c_long, _vp, ctypes.c_float, _vp, _vp]),
     'tf_gemm_tuning': (_c_int, [_c_int]),
     'tf_gemm_table': (_c_int, [_ip, _c_int]),
-    'tf_bcr_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
+    'tf_bcr_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     'tf_bcr_destroy': (_c_int, [_vp]),
     'tf_bcr_factor': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),
     'tf_bcr_status': (_c_int, [_vp, _ip, _vp]),

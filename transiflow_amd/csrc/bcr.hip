@@ -46,6 +46,24 @@ struct Cpl {
   double val;
 };
 
+// Line ordering of the unknowns.  Rows of one line are contiguous in the
+// factor: natural order when lines are x-lines (2D) or planes (3D); for 2D
+// y-lines (line = column i, m = ny dof) the natural index d + dof (i + nx j)
+// maps to i m + j dof + d.
+struct Order {
+  int ylines, nx, dof, m;
+  __host__ __device__ __forceinline__ int to_line(int r) const {
+    if (!ylines) return r;
+    const int d = r % dof, cell = r / dof, i = cell % nx, j = cell / nx;
+    return i * m + j * dof + d;
+  }
+  __host__ __device__ __forceinline__ int to_natural(int q) const {
+    if (!ylines) return q;
+    const int i = q / m, rest = q - i * m, j = rest / dof, d = rest - j * dof;
+    return d + dof * (i + nx * j);
+  }
+};
+
 __global__ __launch_bounds__(256) void k_zero_int(long n, int* __restrict__ p) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) p[e] = 0;
 }
@@ -60,44 +78,46 @@ __global__ __launch_bounds__(256) void k_zero_many(long n, T* const* __restrict_
 // Row-wise couplings: Lr[r] (columns in line(r)-1, as offsets), Ur[r] (line(r)+1).
 // Column-wise couplings: Ut[c] = rows of line(c)-1 coupling into column c,
 //                        Lt[c] = rows of line(c)+1 coupling into column c.
-__global__ __launch_bounds__(256) void k_couplings4(int n, int m, const int* __restrict__ rp,
+__global__ __launch_bounds__(256) void k_couplings4(int n, int m, Order o, const int* __restrict__ rp,
                                                     const int* __restrict__ ci, const double* __restrict__ v,
                                                     int gauge, Cpl* __restrict__ Lr, int* __restrict__ nLr,
                                                     Cpl* __restrict__ Ur, int* __restrict__ nUr,
                                                     Cpl* __restrict__ Ut, int* __restrict__ nUt,
                                                     Cpl* __restrict__ Lt, int* __restrict__ nLt,
                                                     int* __restrict__ err) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;  // natural row
   if (r >= n) return;
-  const int j = r / m;
+  const int q = o.to_line(r);                              // line-major row
+  const int j = q / m;
   int nl = 0, nu = 0;
   if (r != gauge) {
     for (int k = rp[r]; k < rp[r + 1]; ++k) {
       const int c = ci[k];
       if (c == gauge) continue;
-      const int jc = c / m;
+      const int qc = o.to_line(c);
+      const int jc = qc / m;
       if (jc == j) continue;
       if (jc == j - 1) {
-        if (nl < KC) Lr[(long)r * KC + nl] = Cpl{c - jc * m, 0, v[k]};
+        if (nl < KC) Lr[(long)q * KC + nl] = Cpl{qc - jc * m, 0, v[k]};
         else atomicOr(err, 1);
         ++nl;
-        const int s = atomicAdd(&nLt[c], 1);
-        if (s < KC) Lt[(long)c * KC + s] = Cpl{r - j * m, 0, v[k]};
+        const int s = atomicAdd(&nLt[qc], 1);
+        if (s < KC) Lt[(long)qc * KC + s] = Cpl{q - j * m, 0, v[k]};
         else atomicOr(err, 1);
       } else if (jc == j + 1) {
-        if (nu < KC) Ur[(long)r * KC + nu] = Cpl{c - jc * m, 0, v[k]};
+        if (nu < KC) Ur[(long)q * KC + nu] = Cpl{qc - jc * m, 0, v[k]};
         else atomicOr(err, 1);
         ++nu;
-        const int s = atomicAdd(&nUt[c], 1);
-        if (s < KC) Ut[(long)c * KC + s] = Cpl{r - j * m, 0, v[k]};
+        const int s = atomicAdd(&nUt[qc], 1);
+        if (s < KC) Ut[(long)qc * KC + s] = Cpl{q - j * m, 0, v[k]};
         else atomicOr(err, 1);
       } else {
         atomicOr(err, 2);
       }
     }
   }
-  nLr[r] = nl < KC ? nl : KC;
-  nUr[r] = nu < KC ? nu : KC;
+  nLr[q] = nl < KC ? nl : KC;
+  nUr[q] = nu < KC ? nu : KC;
 }
 
 // Sort the column-wise lists by row (the atomic append order is arbitrary).
@@ -120,10 +140,11 @@ __global__ __launch_bounds__(256) void k_sort_cols(int n, Cpl* __restrict__ A, i
 
 // D[j] = dense diagonal block of line j (gauged).  grid (m, N).
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_blocks(int m, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                     const double* __restrict__ v, int gauge, T* __restrict__ D) {
+__global__ __launch_bounds__(256) void k_diag_blocks(int m, Order o, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci, const double* __restrict__ v,
+                                                     int gauge, T* __restrict__ D) {
   const int rl = blockIdx.x, j = blockIdx.y;
-  const long r = (long)j * m + rl;
+  const long r = o.to_natural(j * m + rl);
   T* row = D + ((long)j * m + rl) * m;
   for (int c = threadIdx.x; c < m; c += blockDim.x) row[c] = (T)0;
   __syncthreads();
@@ -134,7 +155,7 @@ __global__ __launch_bounds__(256) void k_diag_blocks(int m, const int* __restric
   for (int k = rp[r] + threadIdx.x; k < rp[r + 1]; k += blockDim.x) {
     const int c = ci[k];
     if (c == gauge) continue;
-    const long cl = (long)c - (long)j * m;
+    const long cl = (long)o.to_line(c) - (long)j * m;
     if (cl >= 0 && cl < m) row[cl] = (T)v[k];
   }
 }
@@ -430,6 +451,16 @@ __global__ __launch_bounds__(256) void k_copy(long n, const V* __restrict__ a, V
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) b[e] = a[e];
 }
 
+// line-major <- natural (gather) and natural <- line-major (scatter)
+template <typename V>
+__global__ __launch_bounds__(256) void k_to_lines(int n, Order o, const V* __restrict__ nat, V* __restrict__ lin) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) lin[q] = nat[o.to_natural(q)];
+}
+template <typename V>
+__global__ __launch_bounds__(256) void k_to_natural(int n, Order o, const V* __restrict__ lin, V* __restrict__ nat) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) nat[o.to_natural(q)] = lin[q];
+}
+
 // ---- host-side schedule ----------------------------------------------------------
 
 struct LevelPlan {
@@ -494,6 +525,8 @@ struct BcrLevelDev {
 
 struct tf_bcr {
   int n, m, N, single;
+  Order order;
+  void* xw;    // n (line-major solution when the order is not natural)
   size_t es;
   void* D;    // N m^2: diagonal blocks, inverted in place (Y)
   void* Lm;   // N m^2: dense lower couplings (levels >= 1)
@@ -566,11 +599,12 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
   hipLaunchKernelGGL(k_zero_int, dim3(4), dim3(256), 0, st, 2L, h->flags);
   hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (long)n, h->nUt);
   hipLaunchKernelGGL(k_zero_int, dim3(256), dim3(256), 0, st, (long)n, h->nLt);
-  hipLaunchKernelGGL(k_couplings4, dim3((n + 255) / 256), dim3(256), 0, st, n, m, rp, ci, v, gauge, h->Lr, h->nLr,
+  hipLaunchKernelGGL(k_couplings4, dim3((n + 255) / 256), dim3(256), 0, st, n, m, h->order, rp, ci, v, gauge, h->Lr,
+                     h->nLr,
                      h->Ur, h->nUr, h->Ut, h->nUt, h->Lt, h->nLt, h->flags);
   hipLaunchKernelGGL(k_sort_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, h->Ut, h->nUt);
   hipLaunchKernelGGL(k_sort_cols, dim3((n + 255) / 256), dim3(256), 0, st, n, h->Lt, h->nLt);
-  hipLaunchKernelGGL(k_diag_blocks<T>, dim3(m, N), dim3(256), 0, st, m, rp, ci, v, gauge, (T*)h->D);
+  hipLaunchKernelGGL(k_diag_blocks<T>, dim3(m, N), dim3(256), 0, st, m, h->order, rp, ci, v, gauge, (T*)h->D);
   for (size_t l = 0; l < h->lev.size(); ++l) {
     BcrLevelDev& L = h->lev[l];
     // 1. invert the eliminated lines' diagonal blocks in place
@@ -626,11 +660,16 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   V* tb = (V*)h->tb;
   TF_PROF_WORK(tf::PROF_SOLVE, (double)h->solve_bytes);
   TF_PROF_BEGIN(tf::PROF_SOLVE, st);
-  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
+  const bool nat = !h->order.ylines;
+  V* xo = nat ? x : (V*)h->xw;   // the kernels work in line-major order
+  if (nat)
+    hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
+  else
+    hipLaunchKernelGGL(k_to_lines<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, r, rw);
   const dim3 blk(256);
   // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
   constexpr int W = 16 / (int)sizeof(T);
-  const int vec = (sizeof(T) == sizeof(V) && m % W == 0 && ((uintptr_t)x & 15) == 0) ? 1 : 0;
+  const int vec = (sizeof(T) == sizeof(V) && m % W == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
   const unsigned gx = (unsigned)((m + 15) / 16);
   for (size_t l = 0; l < h->lev.size(); ++l) {
     BcrLevelDev& L = h->lev[l];
@@ -641,10 +680,11 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     BcrLevelDev& L = h->lev[l];
     hipLaunchKernelGGL((k_bwd_t<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, l == 0 ? 1 : 0, L.e, L.ea, L.eb,
-                       (T* const*)L.Le, (T* const*)L.Ue, h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)x, tb);
+                       (T* const*)L.Le, (T* const*)L.Ue, h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)xo, tb);
     hipLaunchKernelGGL((k_bwd_x<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, (T* const*)L.Ye, (const V*)tb,
-                       x);
+                       xo);
   }
+  if (!nat) hipLaunchKernelGGL(k_to_natural<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, (const V*)xo, x);
   TF_PROF_END(tf::PROF_SOLVE, st);
   TF_LAUNCH_CHECK("bcr solve");
   return 0;
@@ -682,11 +722,16 @@ struct PoolBuilder {
 
 extern "C" {
 
-int tf_bcr_create(tf_bcr** out, int n, int m, int single) {
+// ylines != 0 (2D only): lines are the grid columns i (m = ny dof rows each, nx
+// lines) instead of the rows j; n = nx ny dof.
+int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int ylines) {
   if (!out) return tf::fail("null handle");
   *out = nullptr;
   if (m < 1 || n % m != 0) return tf::fail("line block size must divide n");
+  if (ylines && (nx < 1 || dof < 1 || m % dof != 0 || (long)nx * m != n))
+    return tf::fail("y-line order needs n = nx * m with m = ny * dof");
   tf_bcr* h = new tf_bcr();
+  h->order = Order{ylines ? 1 : 0, nx, dof, m};
   h->n = n;
   h->m = m;
   h->N = n / m;
@@ -715,6 +760,7 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single) {
   TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
   TF_HIP(hipMalloc(&h->rw, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->tb, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->xw, sizeof(double) * (size_t)n));
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -899,7 +945,7 @@ int tf_bcr_destroy(tf_bcr* h) {
   if (!h) return 0;
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
-                  (void*)h->flags, h->rw, h->tb, h->pool})
+                  (void*)h->flags, h->rw, h->tb, h->xw, h->pool})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;

@@ -42,6 +42,7 @@ factorised?) to ``Solver.log`` for measurement.
 import ctypes
 import math
 import time
+import weakref
 
 import numpy
 
@@ -99,17 +100,28 @@ class LineFactor:
     _ABI = {'line': 'tf_bcr', 'thomas': 'tf_linelu'}
 
     def __init__(self, itf, jac, gauge, single=False, kind='line'):
-        m = line_block_size(itf)
         self.name = kind
         self._p = self._ABI[kind]
         h = ctypes.c_void_p()
-        _lib.call(self._p + '_create', ctypes.byref(h), itf.n, m, 1 if single else 0)
+        if kind == 'line':
+            m, ylines = line_layout(itf)
+            _lib.call('tf_bcr_create', ctypes.byref(h), itf.n, m, 1 if single else 0, itf.nx, itf.dof, ylines)
+        else:
+            m = line_block_size(itf)
+            _lib.call(self._p + '_create', ctypes.byref(h), itf.n, m, 1 if single else 0)
         self._h = h
         self.single = single
         self.m = m
+        self.owner = None
         self.refactor(jac, gauge)
 
     def refactor(self, jac, gauge):
+        """Factor ``jac`` into this handle's buffers (the previous owner's factor is gone)."""
+        old = self.owner() if self.owner is not None else None
+        if old is not None and old is not jac and old.prec is self:
+            old.prec = None
+            old.prec_exact = False
+        self.owner = weakref.ref(jac)
         _lib.call(self._p + '_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), stream())
         flags = (ctypes.c_int * 2)()
         _lib.call(self._p + '_status', self._h, flags, stream())
@@ -137,16 +149,24 @@ class LineFactor:
 
 
 def line_block_size(itf):
-    """Rows of one grid line (2D) or grid plane (3D)."""
+    """Rows of one grid line (2D x-line) or grid plane (3D)."""
     if itf.nz == 1:
         return itf.nx * itf.dof
     return itf.nx * itf.ny * itf.dof
 
 
+def line_layout(itf):
+    """(rows per line, y-lines?) of the block cyclic reduction: in 2D the
+    shorter of the x- and y-lines (fewer rows per dense block)."""
+    if itf.nz == 1 and itf.ny < itf.nx:
+        return itf.ny * itf.dof, 1
+    return line_block_size(itf), 0
+
+
 def line_fits(itf, single=False):
     """Block cyclic reduction keeps five dense m x m blocks per line (D, L, U and
     the two elimination multipliers)."""
-    m = line_block_size(itf)
+    m = line_layout(itf)[0]
     nlines = itf.n // m
     return 5 * nlines * m * m * (4 if single else 8) <= LINE_BUDGET_BYTES
 
@@ -295,6 +315,7 @@ class Solver:
         self._krylov = {}
         self._lagged = None   # (precond) reused across Jacobians in mixed precision
         self._buf = None
+        self._factors = {}    # (kind, single) -> LineFactor, refactored in place
 
     def _krylov_for(self, restart, dtype='float64'):
         k = self._krylov.get(dtype)
@@ -311,7 +332,15 @@ class Solver:
         if kind == 'auto':
             kind = 'line' if line_fits(self.itf, single) else 'block-jacobi'
         if kind in ('line', 'thomas'):
-            return LineFactor(self.itf, jac, g, single=single, kind=kind)
+            # one factor handle per (kind, precision), refactored in place: the
+            # dense blocks are GBs, so a new Jacobian reuses the buffers instead
+            # of a fresh allocation (the factor of the previous Jacobian goes)
+            f = self._factors.get((kind, single))
+            if f is None:
+                f = self._factors[(kind, single)] = LineFactor(self.itf, jac, g, single=single, kind=kind)
+            else:
+                f.refactor(jac, g)
+            return f
         if kind == 'block-jacobi':
             return CellBlockJacobi(self.itf, jac, g)
         raise ValueError('unknown preconditioner %r' % kind)
@@ -381,7 +410,7 @@ class Solver:
 
     def _fp64_fallback(self, jac, b, x, opts):
         prec = getattr(jac, 'prec64', None)
-        if prec is None:
+        if prec is None or prec.owner is None or prec.owner() is not jac:
             kind = opts['Preconditioner']
             prec = jac.prec64 = self._build(jac, kind, single=False)
         kry = self._krylov_for(opts['Restart'])
@@ -480,8 +509,8 @@ class Solver:
 
     def line_stats(self):
         """Shape of the line factorisation for this problem (rows per line, lines)."""
-        m = line_block_size(self.itf)
-        return {'m': m, 'lines': self.itf.n // m}
+        m, ylines = line_layout(self.itf)
+        return {'m': m, 'lines': self.itf.n // m, 'ylines': bool(ylines)}
 
     def solve_bordered(self, jac, rhs, rhs2, V, W, C):
         """Bordered solve of SciPy.direct_solve (SciPy.py:51-129,204-258):
