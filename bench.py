@@ -193,6 +193,37 @@ def kernel_sample(n, reps=10):
     log('kernel sample done: %d applies, %d-float copy' % (reps, big))
 
 
+def batch_main(args):
+    """64 cavity instances (square grids drawn from 32..256 step 32, target Re
+    uniform in [100, 1500], numpy default_rng(2)), each Newton at Re = 0 then
+    continuation to its target (ds = 100, tol 1e-8), mixed precision.  One
+    pass runs them one after another on one stream (it also fills the GEMM
+    tile tuner); the timed pass runs them on ``--workers`` concurrent streams.
+    Parity: every instance's final state against its one-stream run."""
+    import numpy
+    import torch
+    from transiflow_amd.batch import c4_instances, run_batch
+    inst = c4_instances()
+    log('batch: %d instances, %d unknowns in total' % (len(inst), sum(3 * i['n'] ** 2 for i in inst)))
+    seq, seq_wall = run_batch(inst, workers=1)
+    log('one stream: %.2f s' % seq_wall)
+    bat, wall = run_batch(inst, workers=args.workers)
+    log('%d streams: %.2f s' % (args.workers, wall))
+    acc = sum(r['accepted'] for r in bat)
+    err = max(float(numpy.linalg.norm(b['x'] - s['x']) / numpy.linalg.norm(s['x'])) for b, s in zip(bat, seq))
+    print(json.dumps({
+        'metric': METRIC, 'value': acc / wall, 'unit': 'steps/s', 'n_gpus': 1, 'steps': acc, 'warmup': 0,
+        'ms_per_step': 1000.0 * wall / acc, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)', 'data': 'synthetic',
+        'config': {'workload': '64 lid-driven cavity 2D instances, n in {32..256 step 32}, Re in U[100, 1500] '
+                               '(default_rng(2)), continuation from Re 0, ds 100, tol 1e-8, %d streams' % args.workers},
+        'one_stream_steps_per_s': acc / seq_wall, 'speedup_vs_one_stream': seq_wall / wall,
+        'max_final_state_rel_diff_vs_one_stream': err,
+        'instances': [{'n': i['n'], 'Re': i['Re'], 'accepted': r['accepted'], 'seconds': r['seconds']}
+                      for i, r in zip(inst, bat)]}), flush=True)
+    del torch
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -204,6 +235,10 @@ def main():
     ap.add_argument('--ds', type=float, default=100.0)
     ap.add_argument('--variants', default='fp64,mixed')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch'],
+                    help="'ldc': one 256x256 cavity continuation (default); 'batch': 64 seeded cavity "
+                         "instances of uneven size and target Reynolds number")
+    ap.add_argument('--workers', type=int, default=8, help='batch workload: concurrent worker streams')
     ap.add_argument('--kernel-sample', action='store_true',
                     help='run only the short launch sequence used for hardware-counter profiles')
     args = ap.parse_args()
@@ -214,6 +249,9 @@ def main():
 
     if args.kernel_sample:
         kernel_sample(args.n)
+        return
+    if args.workload == 'batch':
+        batch_main(args)
         return
 
     import numpy

@@ -117,6 +117,17 @@ int tf_multiaxpy_f32(int n, int k, const float* V, long ldv, const double* coef,
 int tf_normalize_f64(int n, const double* w, double* work, double* nrm_dev, double* out, void* stream);
 int tf_normalize_f32(int n, const float* w, double* work, double* nrm_dev, float* out, void* stream);
 
+/* Native restarted FGMRES (the whole solve in one call; Givens rotations on the host).
+ * single != 0: FP32 basis and operands.  Preconditioner kind 0 none, 1 tf_bcr, 2 tf_linelu,
+ * 3 cell-block inverse (prec = the FP64 inverse blocks, ncell, dof).  The matrix values
+ * have the handle's precision.  Replaces scipy.sparse.linalg.gmres (SciPy.py:343-370). */
+typedef struct tf_krylov tf_krylov;
+int tf_krylov_create(tf_krylov** out, int n, int restart, int single);
+int tf_krylov_destroy(tf_krylov* k);
+int tf_fgmres(tf_krylov* k, const int* row_ptr, const int* col, const void* val, int gauge, int prec_kind,
+              void* prec, int ncell, int dof, const void* b, void* x, double tol, int maxit, int x_is_zero,
+              int* its_out, double* rel_out, void* stream);
+
 /* Cell-block Jacobi preconditioner (dof x dof diagonal blocks of the gauged matrix). */
 int tf_cellblock_setup(int ncell, int dof, const int* row_ptr, const int* col, const double* val,
                        int gauge, double* inv, void* stream);

@@ -74,6 +74,10 @@ SIGNATURES = {
     'tf_adam': (_c_int, [_c_long, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, _c_int, _vp]),
     'tf_to_bf16': (_c_int, [_c_long, _vp, ctypes.c_float, _vp, _vp]),
+    'tf_krylov_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
+    'tf_krylov_destroy': (_c_int, [_vp]),
+    'tf_fgmres': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_double, _c_int,
+                           _c_int, _ip, _dp, _vp]),
     'tf_gemm_tuning': (_c_int, [_c_int]),
     'tf_gemm_table': (_c_int, [_ip, _c_int]),
     'tf_bcr_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),

@@ -32,7 +32,7 @@ ARCH = 'gfx950'
 
 COMMON = ['-O3', '--offload-arch=' + ARCH, '-fPIC', '-std=c++17', '-Wall', '-Wno-unused-function',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-          '-I', CSRC]
+          '-I', CSRC, '-I', os.path.join(os.path.dirname(HERE), 'include')]
 
 SOURCES = {
     'assemble.hip': ['-ffp-contract=off', '-fno-signed-zeros', '-ffinite-math-only'],
@@ -42,6 +42,7 @@ SOURCES = {
     'bcr.hip': [],
     'gemm.hip': [],
     'conv.hip': [],
+    'gmres.hip': [],
 }
 
 

@@ -1,0 +1,226 @@
+// Restarted flexible GMRES on the host side of the library: the whole solve is
+// one C call, so the Python caller (and the interpreter lock) is out of the
+// Arnoldi loop, and concurrent solves on different streams proceed
+// independently.  Right preconditioning with the line factorisation
+// (tf_bcr / tf_linelu) or cell-block Jacobi, CGS2 orthogonalisation with the
+// device kernels of krylov.hip, Givens rotations on the host (one (j+2)-entry
+// read-back per iteration).  Same iteration and stopping rules as the
+// reference's scipy.sparse.linalg.gmres use (SciPy.py:343-370) restated for
+// FGMRES: stop when the residual estimate |g_{j+1}| <= tol ||b||, restart with
+// the true residual b - A x.
+#include "tf_common.h"
+#include "transiflow_hip.h"
+
+#include <cmath>
+#include <vector>
+
+struct tf_krylov {
+  int n, m, single;
+  size_t es;
+  void* V;     // (m+1) x n
+  void* Z;     // m x n
+  void* w;     // n
+  double* h;   // m+2 (device)
+  double* y;   // m+1 (device)
+  double* work;
+  double* h_host;  // m+2 (pinned)
+};
+
+namespace {
+
+struct Ops {
+  const tf_krylov* k;
+  int n;
+  hipStream_t st;
+  char* V(int j) const { return (char*)k->V + k->es * (size_t)n * j; }
+  char* Z(int j) const { return (char*)k->Z + k->es * (size_t)n * j; }
+};
+
+int spmv(const tf_krylov* k, int n, const int* rp, const int* ci, const void* val, int gauge, const void* x, void* y,
+         hipStream_t st) {
+  return k->single ? tf_spmv_f32(n, rp, ci, (const float*)val, (const float*)x, (float*)y, gauge, st)
+                   : tf_spmv_f64(n, rp, ci, (const double*)val, (const double*)x, (double*)y, gauge, st);
+}
+
+int precond(const tf_krylov* k, int kind, void* handle, int ncell, int dof, const void* r, void* z, hipStream_t st) {
+  switch (kind) {
+    case 1:
+      return k->single ? tf_bcr_solve_f32((tf_bcr*)handle, (const float*)r, (float*)z, st)
+                       : tf_bcr_solve_f64((tf_bcr*)handle, (const double*)r, (double*)z, st);
+    case 2:
+      return k->single ? tf_linelu_solve_f32((tf_linelu*)handle, (const float*)r, (float*)z, st)
+                       : tf_linelu_solve_f64((tf_linelu*)handle, (const double*)r, (double*)z, st);
+    case 3:
+      return k->single ? tf_cellblock_apply_f32(ncell, dof, (const double*)handle, (const float*)r, (float*)z, st)
+                       : tf_cellblock_apply_f64(ncell, dof, (const double*)handle, (const double*)r, (double*)z, st);
+    default:
+      return hipMemcpyAsync(z, r, k->es * (size_t)k->n, hipMemcpyDeviceToDevice, st) == hipSuccess
+                 ? 0
+                 : tf::fail("copy preconditioner");
+  }
+}
+
+// ||x|| (host double; synchronises the stream)
+int norm(const tf_krylov* k, int n, const void* x, double* out, hipStream_t st) {
+  if (k->single) {
+    if (int e = tf_multidot_f32(n, 1, (const float*)x, n, (const float*)x, k->work, k->h + k->m + 1, 0, st)) return e;
+  } else {
+    if (int e = tf_multidot_f64(n, 1, (const double*)x, n, (const double*)x, k->work, k->h + k->m + 1, 0, st))
+      return e;
+  }
+  TF_HIP(hipMemcpyAsync(k->h_host + k->m + 1, k->h + k->m + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  TF_HIP(hipStreamSynchronize(st));
+  *out = std::sqrt(k->h_host[k->m + 1]);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
+  if (!out) return tf::fail("null handle");
+  *out = nullptr;
+  if (n < 1 || restart < 1 || restart > tf_krylov_max_basis() - 1) return tf::fail("bad Krylov size");
+  tf_krylov* k = new tf_krylov();
+  k->n = n;
+  k->m = restart;
+  k->single = single;
+  k->es = single ? sizeof(float) : sizeof(double);
+  TF_HIP(hipMalloc(&k->V, k->es * (size_t)n * (restart + 1)));
+  TF_HIP(hipMalloc(&k->Z, k->es * (size_t)n * restart));
+  TF_HIP(hipMalloc(&k->w, k->es * (size_t)n));
+  TF_HIP(hipMalloc(&k->h, sizeof(double) * (restart + 2)));
+  TF_HIP(hipMalloc(&k->y, sizeof(double) * (restart + 1)));
+  TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
+  TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
+  *out = k;
+  return 0;
+}
+
+int tf_krylov_destroy(tf_krylov* k) {
+  if (!k) return 0;
+  for (void* p : {k->V, k->Z, k->w, (void*)k->h, (void*)k->y, (void*)k->work}) (void)hipFree(p);
+  (void)hipHostFree(k->h_host);
+  delete k;
+  return 0;
+}
+
+// Solve (gauged A) x = b by FGMRES with A in CSR (values in the handle's precision),
+// preconditioner kind 0 none / 1 tf_bcr / 2 tf_linelu / 3 cell-block inverse (handle =
+// the FP64 inverse blocks, ncell, dof).  x holds the initial guess (x_is_zero: it is
+// zero and the first residual is b).  its_out / rel_out: iterations and the final
+// relative residual -- the true one, except for a single-precision solve that ends on
+// the iteration limit (the residual estimate then; callers refine in FP64).
+int tf_fgmres(tf_krylov* k, const int* rp, const int* ci, const void* val, int gauge, int prec_kind,
+              void* prec, int ncell, int dof, const void* b, void* x, double tol, int maxit, int x_is_zero,
+              int* its_out, double* rel_out, void* stream) {
+  if (!k || !its_out || !rel_out) return tf::fail("null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = k->n, m = k->m;
+  Ops o{k, n, st};
+  double bnrm = 0.0;
+  if (int e = norm(k, n, b, &bnrm, st)) return e;
+  *its_out = 0;
+  *rel_out = 0.0;
+  if (bnrm == 0.0)
+    return k->single ? tf_fill_f32(n, 0.0, (float*)x, st) : tf_fill_f64(n, 0.0, (double*)x, st);
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yk(m);
+  auto Hij = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
+  int its = 0;
+  double rel = INFINITY;
+  bool first = true;
+  for (;;) {
+    double beta;
+    if (first && x_is_zero) {
+      TF_HIP(hipMemcpyAsync(o.V(0), b, k->es * (size_t)n, hipMemcpyDeviceToDevice, st));
+      beta = bnrm;
+    } else {
+      if (int e = spmv(k, n, rp, ci, val, gauge, x, k->w, st)) return e;
+      if (int e = k->single ? tf_axpby_f32(n, 1.0, (const float*)b, -1.0, (const float*)k->w, (float*)o.V(0), st)
+                            : tf_axpby_f64(n, 1.0, (const double*)b, -1.0, (const double*)k->w, (double*)o.V(0), st))
+        return e;
+      if (int e = norm(k, n, o.V(0), &beta, st)) return e;
+    }
+    rel = beta / bnrm;
+    if (rel <= tol || its >= maxit || (!first && beta == 0.0)) break;
+    first = false;
+    if (int e = k->single ? tf_scale_f32(n, 1.0 / beta, (const float*)o.V(0), (float*)o.V(0), st)
+                          : tf_scale_f64(n, 1.0 / beta, (const double*)o.V(0), (double*)o.V(0), st))
+      return e;
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int kk = 0;
+    bool estimate_exit = false;
+    for (int j = 0; j < m; ++j) {
+      if (int e = precond(k, prec_kind, prec, ncell, dof, o.V(j), o.Z(j), st)) return e;
+      if (int e = spmv(k, n, rp, ci, val, gauge, o.Z(j), k->w, st)) return e;
+      // CGS2: h = V^T w, w -= V h, y = V^T w, w -= V y, h += y
+      if (k->single) {
+        const float* Vf = (const float*)k->V;
+        float* wf = (float*)k->w;
+        if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->h, 0, st)) return e;
+        if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->h, wf, -1.0, st)) return e;
+        if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->y, 0, st)) return e;
+        if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->y, wf, -1.0, st)) return e;
+      } else {
+        const double* Vd = (const double*)k->V;
+        double* wd = (double*)k->w;
+        if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->h, 0, st)) return e;
+        if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->h, wd, -1.0, st)) return e;
+        if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->y, 0, st)) return e;
+        if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->y, wd, -1.0, st)) return e;
+      }
+      if (int e = tf_axpby_f64(j + 1, 1.0, k->h, 1.0, k->y, k->h, st)) return e;
+      if (int e = k->single ? tf_normalize_f32(n, (const float*)k->w, k->work, k->h + j + 1, (float*)o.V(j + 1), st)
+                            : tf_normalize_f64(n, (const double*)k->w, k->work, k->h + j + 1, (double*)o.V(j + 1),
+                                               st))
+        return e;
+      TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+      TF_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i <= j + 1; ++i) Hij(i, j) = k->h_host[i];
+      for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * Hij(i, j) + sn[i] * Hij(i + 1, j);
+        Hij(i + 1, j) = -sn[i] * Hij(i, j) + cs[i] * Hij(i + 1, j);
+        Hij(i, j) = t;
+      }
+      const double d = std::hypot(Hij(j, j), Hij(j + 1, j));
+      if (d == 0.0) {
+        kk = j;
+        break;
+      }
+      cs[j] = Hij(j, j) / d;
+      sn[j] = Hij(j + 1, j) / d;
+      Hij(j, j) = d;
+      Hij(j + 1, j) = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++its;
+      kk = j + 1;
+      if (std::fabs(g[j + 1]) / bnrm <= tol || its >= maxit || !std::isfinite(g[j + 1])) break;
+    }
+    if (kk == 0) break;
+    for (int i = kk - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int c = i + 1; c < kk; ++c) s -= Hij(i, c) * yk[c];
+      yk[i] = s / Hij(i, i);
+    }
+    TF_HIP(hipMemcpyAsync(k->y, yk.data(), sizeof(double) * kk, hipMemcpyHostToDevice, st));
+    if (int e = k->single ? tf_multiaxpy_f32(n, kk, (const float*)k->Z, n, k->y, (float*)x, 1.0, st)
+                          : tf_multiaxpy_f64(n, kk, (const double*)k->Z, n, k->y, (double*)x, 1.0, st))
+      return e;
+    if (its >= maxit && k->single) {
+      rel = std::fabs(g[kk]) / bnrm;
+      estimate_exit = true;
+    }
+    if (estimate_exit) break;
+  }
+  // the host copy of y must stay valid until the multiaxpy has read it
+  TF_HIP(hipStreamSynchronize(st));
+  *its_out = its;
+  *rel_out = rel;
+  return 0;
+}
+
+}  // extern "C"

@@ -33,7 +33,7 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Restart'``          Krylov basis size (30);
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
-                       preconditioner is refactored (8).
+                       preconditioner is refactored (5).
 
 Every solve appends (iterations, relative residual, preconditioner name,
 factorised?) to ``Solver.log`` for measurement.
@@ -64,7 +64,7 @@ def _opts(parameters):
         'Tolerance': float(o.get('Tolerance', 1e-10)),
         'Restart': int(o.get('Restart', 30)),
         'Maximum Iterations': int(o.get('Maximum Iterations', 300)),
-        'Reuse Limit': int(o.get('Reuse Limit', 8)),
+        'Reuse Limit': int(o.get('Reuse Limit', 5)),
         'Inner Tolerance': float(o.get('Inner Tolerance', 1e-3)),
         'Inner Iterations': int(o.get('Inner Iterations', 10)),
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
@@ -91,6 +91,10 @@ class CellBlockJacobi:
     def apply32(self, r, out):
         _lib.call('tf_cellblock_apply_f32', self.ncell, self.dof, ptr(self.inv), ptr(r), ptr(out), stream())
         return out
+
+    def native(self):
+        """(kind, handle, ncell, dof) for tf_fgmres."""
+        return 3, ptr(self.inv), self.ncell, self.dof
 
 
 class LineFactor:
@@ -137,6 +141,10 @@ class LineFactor:
     def apply32(self, r, out):
         _lib.call(self._p + '_solve_f32', self._h, ptr(r), ptr(out), stream())
         return out
+
+    def native(self):
+        """(kind, handle, ncell, dof) for tf_fgmres."""
+        return (1 if self._p == 'tf_bcr' else 2), self._h, 0, 0
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -288,6 +296,35 @@ class Krylov:
                 return its, abs(g[k]) / bnrm
 
 
+class NativeKrylov:
+    """The same FGMRES as :class:`Krylov`, run inside the library (tf_fgmres):
+    one call per solve, no interpreter in the Arnoldi loop."""
+
+    def __init__(self, n, restart, dtype='float64'):
+        self.n, self.m, self.dtype = n, restart, dtype
+        h = ctypes.c_void_p()
+        _lib.call('tf_krylov_create', ctypes.byref(h), n, restart, 1 if dtype == 'float32' else 0)
+        self._h = h
+
+    def solve(self, jac, val, gauge, prec, b, x, tol, maxit, x_is_zero=False):
+        kind, handle, ncell, dof = prec.native()
+        its = ctypes.c_int()
+        rel = ctypes.c_double()
+        _lib.call('tf_fgmres', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(val), int(gauge), kind, handle, ncell,
+                  dof, ptr(b), ptr(x), float(tol), int(maxit), 1 if x_is_zero else 0, ctypes.byref(its),
+                  ctypes.byref(rel), stream())
+        return its.value, rel.value
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            try:
+                _lib.load().tf_krylov_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
 def _back_substitute(R, g):
     k = len(g)
     y = numpy.zeros(k)
@@ -317,10 +354,11 @@ class Solver:
         self._buf = None
         self._factors = {}    # (kind, single) -> LineFactor, refactored in place
 
-    def _krylov_for(self, restart, dtype='float64'):
-        k = self._krylov.get(dtype)
+    def _krylov_for(self, restart, dtype='float64', native=False):
+        key = (dtype, native)
+        k = self._krylov.get(key)
         if k is None or k.m != restart:
-            k = self._krylov[dtype] = Krylov(self.itf.n, restart, dtype)
+            k = self._krylov[key] = (NativeKrylov if native else Krylov)(self.itf.n, restart, dtype)
         return k
 
     def gauge(self):
@@ -391,6 +429,10 @@ class Solver:
     def _solve_fp64(self, jac, b, x, opts):
         g = self.gauge()
         prec, fresh = self.preconditioner(jac, opts)
+        if hasattr(prec, 'native'):
+            kry = self._krylov_for(opts['Restart'], native=True)
+            its, rel = kry.solve(jac, jac.val, g, prec, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+            return its, rel, prec, fresh
         kry = self._krylov_for(opts['Restart'])
 
         def matvec(v, out):
@@ -419,7 +461,11 @@ class Solver:
         def matvec(v, out):
             jac.matvec(v, out=out, gauge=g)
 
-        its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+        if hasattr(prec, 'native'):
+            nk = self._krylov_for(opts['Restart'], native=True)
+            its, rel = nk.solve(jac, jac.val, g, prec, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+        else:
+            its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
         return its, rel, prec, True
 
     def _buffers(self, n):
@@ -487,8 +533,13 @@ class Solver:
             else:
                 _lib.call('tf_f64_to_f32', n, ptr(b), ptr(r32), st)
             _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
-            its, est = kry.solve(matvec32, inner_prec().apply32, r32, d32, opts['Inner Tolerance'], inner_max,
-                                 x_is_zero=True)
+            p_inner = inner_prec()
+            if p_inner is prec and hasattr(prec, 'native'):
+                nk = self._krylov_for(opts['Restart'], 'float32', native=True)
+                its, est = nk.solve(jac, val32, g, prec, r32, d32, opts['Inner Tolerance'], inner_max, x_is_zero=True)
+            else:
+                its, est = kry.solve(matvec32, p_inner.apply32, r32, d32, opts['Inner Tolerance'], inner_max,
+                                     x_is_zero=True)
             self.trace.append((its, est, rel))
             total += its
             _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)

@@ -14,6 +14,7 @@ the object like the reference caches its LU on the matrix (SciPy.py:134-152).
 """
 
 import ctypes
+import threading
 import numbers
 
 import numpy
@@ -45,18 +46,22 @@ def ptr(t):
 
 
 class _Workspace:
-    """Per-process scratch for reductions (one per device)."""
-    _buf = {}
+    """Scratch for reductions, one per thread and device: threads that run
+    instances on their own streams never share it."""
+    _local = threading.local()
 
     @classmethod
     def get(cls):
         T = torch()
         dev = T.cuda.current_device()
-        buf = cls._buf.get(dev)
+        bufs = getattr(cls._local, 'bufs', None)
+        if bufs is None:
+            bufs = cls._local.bufs = {}
+        buf = bufs.get(dev)
         if buf is None:
             nwork = _lib.load().tf_reduce_work_size()
             buf = T.empty(nwork + 8, dtype=T.float64, device='cuda')
-            cls._buf[dev] = buf
+            bufs[dev] = buf
         return buf
 
 
