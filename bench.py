@@ -46,15 +46,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_continuation(tf, torch, n, precision, target, ds, maxit, restart=30):
-    params = {'HIP Solver': {'Precision': precision, 'Restart': restart}}
-    itf = tf.Interface(params, n, n)
+def workload(name, n, target, ds):
+    """Continuation workloads: 'ldc' the 2D cavity (n x n, Re 0 -> target, the
+    default and the headline), 'rb' Rayleigh-Benard 2D 512x128 (Pr = 1, Biot 1,
+    X-max 4, seeded 1e-3 perturbation of T, Ra 0 -> 5000), 'ldc3d' the 3D
+    cavity at 32^3 (Re 0 -> 1000)."""
+    if name == 'rb':
+        return {'params': {'Problem Type': 'Rayleigh-Benard', 'Prandtl Number': 1.0, 'Biot Number': 1.0,
+                           'X-max': 4.0, 'Rayleigh Number': 0.0},
+                'shape': (512, 128, 1), 'param': 'Rayleigh Number', 'target': 5000.0, 'ds': 100.0, 'perturb': True,
+                'label': 'Rayleigh-Benard 2D 512x128 (n=262144), Pr=1, Biot=1, X-max=4, T perturbed by '
+                         '1e-3*U(-1,1) (default_rng(1)), Newton at Ra=0, continuation Ra 0->5000, ds=100, tol 1e-8'}
+    if name == 'ldc3d':
+        return {'params': {}, 'shape': (32, 32, 32), 'param': 'Reynolds Number', 'target': 1000.0, 'ds': 100.0,
+                'perturb': False,
+                'label': 'lid-driven cavity 3D 32^3 (n=131072), Newton from zero at Re=0, continuation Re 0->1000, '
+                         'ds=100, tol 1e-8'}
+    return {'params': {}, 'shape': (n, n, 1), 'param': 'Reynolds Number', 'target': target, 'ds': ds,
+            'perturb': False,
+            'label': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength continuation '
+                     'Re 0->%g, ds=%g, Newton tol 1e-8' % (n, n, 3 * n * n, target, ds)}
+
+
+def run_continuation(tf, torch, spec, precision, maxit, restart=30):
+    import numpy
+    params = dict(spec['params'])
+    params['HIP Solver'] = {'Precision': precision, 'Restart': restart}
+    itf = tf.Interface(params, *spec['shape'])
     cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
+    start = itf.vector()
+    if spec['perturb']:
+        h = numpy.zeros(itf.n)
+        h[itf.dof - 1::itf.dof] = 1e-3 * numpy.random.default_rng(1).uniform(-1, 1, itf.n // itf.dof)
+        start = itf.vector_from_array(h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x0 = cont.newton(itf.vector(), tol=1e-8)
+    x0 = cont.newton(start, tol=1e-8)
     newton0 = cont.newton_iterations
-    x, mu = cont.continuation(x0, 'Reynolds Number', 0, target, ds, maxit=maxit)
+    x, mu = cont.continuation(x0, spec['param'], 0, spec['target'], spec['ds'], maxit=maxit)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     acc = sum(1 for h in cont.history if h[0] == 'accepted')
@@ -116,14 +145,14 @@ PROFILED = {
 }
 
 
-def profiled_run(tf, torch, n, precision, target, ds, maxit):
+def profiled_run(tf, torch, spec, precision, maxit):
     """Repeat a timed continuation with the HIP-event profiler on; returns
     per-kernel (launches, ms, work)."""
     from transiflow_amd import _lib
     lib = _lib.load()
     lib.tf_prof_reset()
     lib.tf_prof_enable(1)
-    r = run_continuation(tf, torch, n, precision, target, ds, maxit)
+    r = run_continuation(tf, torch, spec, precision, maxit)
     torch.cuda.synchronize()
     lib.tf_prof_enable(0)
     out = {}
@@ -235,9 +264,10 @@ def main():
     ap.add_argument('--ds', type=float, default=100.0)
     ap.add_argument('--variants', default='fp64,mixed')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch'],
+    ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch', 'rb', 'ldc3d'],
                     help="'ldc': one 256x256 cavity continuation (default); 'batch': 64 seeded cavity "
-                         "instances of uneven size and target Reynolds number")
+                         "instances of uneven size and target Reynolds number; 'rb': Rayleigh-Benard 2D "
+                         "512x128; 'ldc3d': 3D cavity 32^3")
     ap.add_argument('--workers', type=int, default=8, help='batch workload: concurrent worker streams')
     ap.add_argument('--kernel-sample', action='store_true',
                     help='run only the short launch sequence used for hardware-counter profiles')
@@ -259,16 +289,17 @@ def main():
     import transiflow_amd as tf
 
     maxit = args.steps if args.steps > 0 else 1000
+    spec = workload(args.workload, args.n, args.target, args.ds)
     if args.warmup > 0:
         for var in args.variants.split(','):
             # also fills the GEMM tile tuner for this precision's shapes
-            log('warm-up (%s): %d step(s) at %dx%d' % (var, args.warmup, args.n, args.n))
-            run_continuation(tf, torch, args.n, var, args.target, args.ds, args.warmup)
+            log('warm-up (%s): %d step(s), %s' % (var, args.warmup, spec['label']))
+            run_continuation(tf, torch, spec, var, args.warmup)
 
     results = {}
     for var in args.variants.split(','):
         log('timed run: %s' % var)
-        results[var] = run_continuation(tf, torch, args.n, var, args.target, args.ds, maxit)
+        results[var] = run_continuation(tf, torch, spec, var, maxit)
         r = results[var]
         log('  %s: %.2f s, %d accepted steps, %.3f steps/s, %d factorizations, %d GMRES its'
             % (var, r['seconds'], r['accepted'], r['steps_per_s'], r['factorizations'], r['gmres_iterations']))
@@ -287,7 +318,7 @@ def main():
     log('kernel timing sample')
     asm = assembly_roofline(tf, torch, hr['itf'], hr['x'])
     log('profiled repeat: %s' % head)
-    pr, kt = profiled_run(tf, torch, args.n, head, args.target, args.ds, maxit)
+    pr, kt = profiled_run(tf, torch, spec, head, maxit)
     pid, (calls, ms, work) = max(kt.items(), key=lambda kv: kv[1][1])
     name, bound, wunit = PROFILED[pid]
     single = head == 'mixed'
@@ -316,7 +347,7 @@ def main():
                                      for k, (c, t, w) in kt.items()}}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == 'ldc':
         log('cpu baseline (oracle, bounded sample)')
         iters_total = hr['corrector_iterations'] + hr['initial_newton_iterations'] + 1
         cpu = cpu_baseline(args.n, hr['x'].numpy(), float(hr['mu']), iters_total, hr['accepted'])
@@ -335,9 +366,7 @@ def main():
         'dtype': 'f64' if head == 'fp64' else 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
         'data': 'synthetic',
         'config': {
-            'workload': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength '
-                        'continuation Re 0->%g, ds=%g, Newton tol 1e-8' % (args.n, args.n, 3 * args.n ** 2,
-                                                                         args.target, args.ds),
+            'workload': spec['label'],
             'variant': head,
             'parallelism': 'single GPU',
         },
