@@ -158,6 +158,10 @@ int tf_bcr_factor(tf_bcr* h, const int* row_ptr, const int* col, const double* v
 int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
 int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
+/* on != 0: the apply is captured once (per vector type) into a hipGraph and replayed with
+ * one launch plus two staging copies (no reference counterpart; launch overhead on small
+ * grids and batches). */
+int tf_bcr_set_graph(tf_bcr* h, int on);
 
 /* ---- learned preconditioner: convolutions on the cell grid (no reference counterpart) ----
  * Activations cell-major [ncell][C] in BF16 (uint16 storage); layer weights in the

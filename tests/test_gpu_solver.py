@@ -178,3 +178,37 @@ def test_bordered_continuation_matches_oracle(hip):
     numpy.testing.assert_allclose(mh, mo, rtol=1e-8, atol=1e-8)
     assert numpy.linalg.norm(xh - xo) / numpy.linalg.norm(xo) <= 1e-8
     assert abs(muh - muo) <= 1e-8
+
+
+@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1)])
+def test_graph_apply_matches_eager(hip, shape):
+    """The hipGraph replay of the line-factor apply gives the eager apply's bits:
+    several replays interleaved with eager applies, then again after the handle
+    is refactored with another Jacobian (the graph keeps its pointers)."""
+    import torch
+    itf = hip.Interface({'Reynolds Number': 500}, *shape)
+    n = itf.n
+    rng = numpy.random.default_rng(31)
+    states = [itf.vector_from_array(rng.uniform(-1, 1, n) * 0.1) for _ in range(2)]
+    r = torch.from_numpy(rng.uniform(-1, 1, n)).cuda()
+    r32 = r.float()
+    for single in (False, True):
+        P = None
+        for state in states:
+            J = itf.jacobian(state)
+            P = itf.solver._build(J, 'line', single)
+            outs = {}
+            for graph in (False, True, False, True, True):
+                P.set_graph(graph)
+                z = torch.empty_like(r)
+                z32 = torch.empty_like(r32)
+                P.apply(r, z)
+                P.apply32(r32, z32)
+                torch.cuda.synchronize()
+                outs.setdefault(graph, []).append((z.cpu().numpy(), z32.cpu().numpy()))
+            ref64, ref32 = outs[False][0]
+            assert numpy.isfinite(ref64).all()
+            for z64, z32 in outs[False][1:] + outs[True]:
+                assert numpy.array_equal(z64, ref64)
+                assert numpy.array_equal(z32, ref32)
+        P.set_graph(False)

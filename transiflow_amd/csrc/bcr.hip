@@ -523,10 +523,21 @@ struct BcrLevelDev {
   void **z_fa, **z_fb;                // L_k (U_k) that become zero
 };
 
+struct BcrGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct tf_bcr {
   int n, m, N, single;
   Order order;
   void* xw;    // n (line-major solution when the order is not natural)
+  // captured apply: gin -> gout replayed as one graph launch (per vector type)
+  int use_graph;
+  void* gin;
+  void* gout;
+  hipStream_t cap;
+  BcrGraph g64, g32;
   size_t es;
   void* D;    // N m^2: diagonal blocks, inverted in place (Y)
   void* Lm;   // N m^2: dense lower couplings (levels >= 1)
@@ -690,6 +701,41 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   return 0;
 }
 
+// The apply captured once per vector type on a private stream (gin -> gout) and
+// replayed with one graph launch; the factor refactors in place, so the
+// captured pointers stay valid.  The graph holds kernel nodes only.
+template <typename T, typename V>
+int solve_graph(tf_bcr* h, const V* r, V* x, hipStream_t st) {
+  BcrGraph& gs = sizeof(V) == 8 ? h->g64 : h->g32;
+  if (!gs.exec) {
+    // non-blocking: a capture must not order against the legacy stream that other
+    // threads use for synchronous copies
+    if (!h->cap) TF_HIP(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+    TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+    const int e = solve_impl<T, V>(h, (const V*)h->gin, (V*)h->gout, h->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->cap, &g);
+    if (e) {
+      if (g) (void)hipGraphDestroy(g);
+      return e;
+    }
+    TF_HIP(ce);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return tf::fail_hip(ie, "hipGraphInstantiate (line-factor apply)");
+    }
+    gs.graph = g;
+    gs.exec = ex;
+  }
+  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, (V*)h->gin);
+  TF_HIP(hipGraphLaunch(gs.exec, st));
+  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, (const V*)h->gout, x);
+  TF_LAUNCH_CHECK("bcr graph apply");
+  return 0;
+}
+
 }  // namespace
 
 // ---- C ABI -----------------------------------------------------------------------
@@ -761,6 +807,8 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
   TF_HIP(hipMalloc(&h->rw, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->tb, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->xw, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->gin, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->gout, sizeof(double) * (size_t)n));
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -943,6 +991,13 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
 
 int tf_bcr_destroy(tf_bcr* h) {
   if (!h) return 0;
+  for (BcrGraph* g : {&h->g64, &h->g32}) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+  }
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  (void)hipFree(h->gin);
+  (void)hipFree(h->gout);
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
                   (void*)h->flags, h->rw, h->tb, h->xw, h->pool})
@@ -971,15 +1026,27 @@ int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream) {
   return 0;
 }
 
+// on != 0: applies replay a captured graph (one launch instead of one per
+// kernel); off while the kernel profiler is on.
+int tf_bcr_set_graph(tf_bcr* h, int on) {
+  if (!h) return tf::fail("null handle");
+  h->use_graph = on ? 1 : 0;
+  return 0;
+}
+
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream) {
   if (!h) return tf::fail("null handle");
   hipStream_t st = (hipStream_t)stream;
+  if (h->use_graph && !tf::prof_on())
+    return h->single ? solve_graph<float, double>(h, r, x, st) : solve_graph<double, double>(h, r, x, st);
   return h->single ? solve_impl<float, double>(h, r, x, st) : solve_impl<double, double>(h, r, x, st);
 }
 
 int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream) {
   if (!h) return tf::fail("null handle");
   hipStream_t st = (hipStream_t)stream;
+  if (h->use_graph && !tf::prof_on())
+    return h->single ? solve_graph<float, float>(h, r, x, st) : solve_graph<double, float>(h, r, x, st);
   return h->single ? solve_impl<float, float>(h, r, x, st) : solve_impl<double, float>(h, r, x, st);
 }
 

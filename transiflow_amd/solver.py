@@ -24,6 +24,8 @@ Options (``parameters['HIP Solver']``, all optional):
                        solves stall near 1e-5 on these Jacobians, so the FP64
                        refinement does the rest);
 ``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
+``'Apply Graph'``      replay the line-factor apply as one captured hipGraph
+                       (True);
 ``'Learned Network'``  (mixed) a ``learned.ConvNet`` (or the path of its saved
                        weights): the inner preconditioner becomes the line
                        factor followed by the judged network correction
@@ -69,6 +71,7 @@ def _opts(parameters):
         'Inner Iterations': int(o.get('Inner Iterations', 10)),
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
         'Learned Network': o.get('Learned Network'),
+        'Apply Graph': bool(o.get('Apply Graph', True)),
     }
 
 
@@ -145,6 +148,12 @@ class LineFactor:
     def native(self):
         """(kind, handle, ncell, dof) for tf_fgmres."""
         return (1 if self._p == 'tf_bcr' else 2), self._h, 0, 0
+
+    def set_graph(self, on):
+        """Replay the apply as one captured hipGraph (block cyclic reduction only)."""
+        if self._p == 'tf_bcr':
+            _lib.call('tf_bcr_set_graph', self._h, 1 if on else 0)
+            self.graph = bool(on)
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -366,6 +375,12 @@ class Solver:
         return itf.pressure_row if (itf.dof > itf.dim and itf.pressure_row is not None) else -1
 
     def _build(self, jac, kind, single):
+        f = self._build_prec(jac, kind, single)
+        if hasattr(f, 'set_graph'):
+            f.set_graph(_opts(self.itf.parameters)['Apply Graph'])
+        return f
+
+    def _build_prec(self, jac, kind, single):
         g = self.gauge()
         if kind == 'auto':
             kind = 'line' if line_fits(self.itf, single) else 'block-jacobi'
