@@ -338,8 +338,8 @@ __global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ 
 // ---- solve ----------------------------------------------------------------------
 //
 // Every solve kernel is a batch of row-times-vector products over m-long rows.
-// A 16-lane group owns one row and reads it with 16-byte loads when the row
-// length and the vectors allow it (16 rows per 256-thread block).
+// A group of lanes owns one row and reads it with 16-byte loads when the row
+// length and the vectors allow it (see RowGroup for the group size).
 
 template <typename T>
 struct Vec16;
@@ -354,10 +354,18 @@ struct Vec16<double> {
   static constexpr int W = 2;
 };
 
-// sum over c of a[c] x[c] by the 16-lane group g (the result is in every lane of the group)
+// 16 lanes per row (measured: 8-lane groups for FP32 rows are slower).
+template <typename T>
+struct RowGroup {
+  static constexpr int G = 16;
+  static constexpr int ROWS = 256 / G;
+};
+
+// sum over c of a[c] x[c] by the G-lane group g (the result is in every lane of the group)
 template <typename T, typename V>
 __device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g,
                                             bool vec) {
+  constexpr int G = RowGroup<T>::G;
   double acc = 0.0;
   if constexpr (sizeof(T) == sizeof(V)) {
     if (vec) {
@@ -365,18 +373,19 @@ __device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __
       constexpr int W = Vec16<T>::W;
       const VT* av = (const VT*)a;
       const VT* xv = (const VT*)x;
-      for (int c = g; c < m / W; c += 16) {
+#pragma unroll 4
+      for (int c = g; c < m / W; c += G) {
         const VT p = av[c], q = xv[c];
 #pragma unroll
         for (int w = 0; w < W; ++w) acc += (double)p[w] * (double)q[w];
       }
     } else {
-      for (int c = g; c < m; c += 16) acc += (double)a[c] * (double)x[c];
+      for (int c = g; c < m; c += G) acc += (double)a[c] * (double)x[c];
     }
   } else {
-    for (int c = g; c < m; c += 16) acc += (double)a[c] * (double)x[c];
+    for (int c = g; c < m; c += G) acc += (double)a[c] * (double)x[c];
   }
-  for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
   return acc;
 }
 
@@ -386,9 +395,10 @@ __global__ __launch_bounds__(256) void k_fwd(int m, int vec, const int* __restri
                                              const int* __restrict__ e1, const int* __restrict__ e2,
                                              T* const* __restrict__ P1p, T* const* __restrict__ P2p,
                                              V* __restrict__ rw) {
+  constexpr int G = RowGroup<T>::G;
   const int b = blockIdx.y;
-  const int g = threadIdx.x & 15;
-  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   double acc = 0.0;
@@ -409,9 +419,10 @@ __global__ __launch_bounds__(256) void k_bwd_t(int m, int vec, int sparse, const
                                                const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
                                                const V* __restrict__ rw, const V* __restrict__ x,
                                                V* __restrict__ tbuf) {
+  constexpr int G = RowGroup<T>::G;
   const int bi = blockIdx.y;
-  const int g = threadIdx.x & 15;
-  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long row = (long)e_line[bi] * m + rr;
@@ -436,9 +447,10 @@ template <typename T, typename V>
 __global__ __launch_bounds__(256) void k_bwd_x(int m, int vec, const int* __restrict__ e_line,
                                                T* const* __restrict__ Yp, const V* __restrict__ tbuf,
                                                V* __restrict__ x) {
+  constexpr int G = RowGroup<T>::G;
   const int bi = blockIdx.y;
-  const int g = threadIdx.x & 15;
-  const int r = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
@@ -681,7 +693,7 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
   constexpr int W = 16 / (int)sizeof(T);
   const int vec = (sizeof(T) == sizeof(V) && m % W == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
-  const unsigned gx = (unsigned)((m + 15) / 16);
+  const unsigned gx = (unsigned)((m + RowGroup<T>::ROWS - 1) / RowGroup<T>::ROWS);
   for (size_t l = 0; l < h->lev.size(); ++l) {
     BcrLevelDev& L = h->lev[l];
     if (L.nk)
