@@ -96,6 +96,14 @@ int tf_spmv_f64(int n, const int* row_ptr, const int* col, const double* val, co
 int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, const float* x,
                 float* y, int gauge, void* stream);
 int tf_set_entry_f64(double* x, long i, double v, void* stream);
+
+/* C = alpha A + beta B for CSR matrices (scipy sparse addition, as used by
+ * TimeIntegration._newton, TimeIntegration.py:56: jacobian(x) - mass / (theta dt)).
+ * A's entries keep their order; B's columns missing in A's row are appended. */
+int tf_csr_add_count(int n, const int* ra, const int* ca, const int* rb, const int* cb, int* cnt, int* rc,
+                     int* nnz_host, void* stream);
+int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, double alpha, const int* rb,
+                    const int* cb, const double* vb, double beta, const int* rc, int* cc, double* vc, void* stream);
 int tf_fill_f64(long n, double v, double* x, void* stream);
 /* FP32 vectors of the mixed-precision inner solve (no reference counterpart). */
 int tf_fill_f32(long n, double v, float* x, void* stream);

@@ -212,3 +212,37 @@ def test_graph_apply_matches_eager(hip, shape):
                 assert numpy.array_equal(z64, ref64)
                 assert numpy.array_equal(z32, ref32)
         P.set_graph(False)
+
+
+def test_csr_combination_matches_scipy(hip):
+    """jacobian(x) - mass / (theta dt) on the device equals scipy's sparse
+    arithmetic on the same matrices (TimeIntegration.py:56)."""
+    itf = hip.Interface({'Reynolds Number': 150}, 12, 10)
+    rng = numpy.random.default_rng(41)
+    J = itf.jacobian(itf.vector_from_array(rng.uniform(-1, 1, itf.n)))
+    M = itf.mass_matrix()
+    C = J - M / 0.37
+    D = 2.5 * J + M
+    ref_C = J.to_scipy() - M.to_scipy() / 0.37
+    ref_D = 2.5 * J.to_scipy() + M.to_scipy()
+    for got, ref in ((C, ref_C), (D, ref_D)):
+        diff = abs(got.to_scipy() - ref)
+        assert diff.max() <= 1e-14 * abs(ref).max()
+
+
+@pytest.mark.parametrize('theta', [1.0, 0.5])
+def test_time_integration_matches_oracle(hip, theta):
+    """Theta method (TimeIntegration.py:38-113) on the HIP backend against the same
+    driver on the oracle backend: LDC 16x16, Re 100, from rest, dt 0.05, 4 steps."""
+    from oracle.interface import Interface as OracleInterface
+    out = {}
+    for name, itf in (('hip', hip.Interface({'Reynolds Number': 100}, 16, 16)),
+                      ('oracle', OracleInterface({'Reynolds Number': 100}, 16, 16, 1, 2, 3))):
+        ti = hip.TimeIntegration(itf, theta=theta, newton_tolerance=1e-9, printer=None)
+        x, t = ti.integration(itf.vector(), 0.05, 0.2)
+        out[name] = (numpy.asarray(itf.array_from_vector(x) if name == 'hip' else x), t, ti.newton_iterations)
+    (xh, th, ih), (xo, to, io) = out['hip'], out['oracle']
+    err = numpy.linalg.norm(xh - xo) / numpy.linalg.norm(xo)
+    print('theta %.1f: steps %d, Newton its %s / %s, rel err %.2e' % (theta, len(ih), ih, io, err))
+    assert th == to and ih == io
+    assert err <= 1e-8

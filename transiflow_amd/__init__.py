@@ -5,12 +5,14 @@ boundary_conditions=None, backend='HIP')`` mirrors ``transiflow.Interface``
 (``transiflow/interface/create.py:4-68``) and returns the HIP backend, whose
 ``rhs``, ``jacobian``, ``mass_matrix`` and ``solve`` run as hand-written HIP
 kernels on gfx950 behind the C ABI of ``include/transiflow_hip.h``.
-``Continuation`` restates ``transiflow.Continuation``.
+``Continuation`` and ``TimeIntegration`` restate ``transiflow.Continuation``
+and ``transiflow.TimeIntegration``.
 """
 
 from .continuation import Continuation, norm
+from .time_integration import TimeIntegration
 
-__all__ = ['Interface', 'create', 'Continuation', 'norm']
+__all__ = ['Interface', 'create', 'Continuation', 'TimeIntegration', 'norm']
 
 
 def create(parameters, nx, ny, nz=1, dim=None, dof=None, boundary_conditions=None, backend='HIP'):

@@ -156,6 +156,61 @@ __global__ __launch_bounds__(256) void k_fill(long n, double v, T* __restrict__ 
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] = v;
 }
 
+// C = alpha A + beta B (CSR), row by row: A's entries in their order, B's
+// entries added where A has the column, otherwise appended in B's order.
+__global__ __launch_bounds__(256) void k_csr_add_count(int n, const int* __restrict__ ra, const int* __restrict__ ca,
+                                                       const int* __restrict__ rb, const int* __restrict__ cb,
+                                                       int* __restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int c = ra[r + 1] - ra[r];
+  for (int q = rb[r]; q < rb[r + 1]; ++q) {
+    bool found = false;
+    for (int p = ra[r]; p < ra[r + 1] && !found; ++p) found = ca[p] == cb[q];
+    if (!found) ++c;
+  }
+  cnt[r] = c;
+}
+
+__global__ void k_exclusive_scan_serial(int n, const int* __restrict__ cnt, int* __restrict__ rp) {
+  // one thread: n is the row count of one matrix (sub-millisecond at these sizes)
+  int s = 0;
+  for (int r = 0; r < n; ++r) {
+    rp[r] = s;
+    s += cnt[r];
+  }
+  rp[n] = s;
+}
+
+__global__ __launch_bounds__(256) void k_csr_add_fill(int n, const int* __restrict__ ra, const int* __restrict__ ca,
+                                                      const double* __restrict__ va, double alpha,
+                                                      const int* __restrict__ rb, const int* __restrict__ cb,
+                                                      const double* __restrict__ vb, double beta,
+                                                      const int* __restrict__ rc, int* __restrict__ cc,
+                                                      double* __restrict__ vc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int o = rc[r];
+  const int a0 = ra[r], a1 = ra[r + 1];
+  for (int p = a0; p < a1; ++p) {
+    cc[o + p - a0] = ca[p];
+    vc[o + p - a0] = alpha * va[p];
+  }
+  int tail = o + (a1 - a0);
+  for (int q = rb[r]; q < rb[r + 1]; ++q) {
+    int hit = -1;
+    for (int p = a0; p < a1 && hit < 0; ++p)
+      if (ca[p] == cb[q]) hit = p;
+    if (hit >= 0) {
+      vc[o + hit - a0] += beta * vb[q];
+    } else {
+      cc[tail] = cb[q];
+      vc[tail] = beta * vb[q];
+      ++tail;
+    }
+  }
+}
+
 int grid_for(long n) {
   long g = (n + 255) / 256;
   return (int)(g < 4096 ? (g < 1 ? 1 : g) : 4096);
@@ -211,6 +266,29 @@ int tf_prof_work(int id, double* work_total) {
   tf::ProfState& s = tf::ps();
   std::lock_guard<std::mutex> g(s.mu);
   *work_total = s.work[id];
+  return 0;
+}
+
+// C = alpha A + beta B.  count writes rc[0..n] (rc[n] = nnz of C) and nnz_host after a
+// stream sync, using cnt (n ints of scratch); fill writes cc, vc.
+int tf_csr_add_count(int n, const int* ra, const int* ca, const int* rb, const int* cb, int* cnt, int* rc,
+                     int* nnz_host, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_csr_add_count, dim3((n + 255) / 256), dim3(256), 0, st, n, ra, ca, rb, cb, cnt);
+  hipLaunchKernelGGL(k_exclusive_scan_serial, dim3(1), dim3(1), 0, st, n, (const int*)cnt, rc);
+  TF_LAUNCH_CHECK("csr add count");
+  if (nnz_host) {
+    TF_HIP(hipMemcpyAsync(nnz_host, rc + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    TF_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, double alpha, const int* rb,
+                    const int* cb, const double* vb, double beta, const int* rc, int* cc, double* vc, void* stream) {
+  hipLaunchKernelGGL(k_csr_add_fill, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ra, ca, va, alpha,
+                     rb, cb, vb, beta, rc, cc, vc);
+  TF_LAUNCH_CHECK("csr add fill");
   return 0;
 }
 

@@ -265,3 +265,49 @@ class DeviceCsrMatrix:
         if isinstance(x, DeviceVector):
             return DeviceVector(self.matvec(x.data))
         return self.to_scipy() @ numpy.asarray(x)
+
+    # -- scipy-like arithmetic (TimeIntegration.py:56: jacobian(x) - mass / (theta dt)) --
+    def _combine(self, alpha, other, beta):
+        T = torch()
+        st = stream()
+        cnt = T.empty(self.n, dtype=T.int32, device='cuda')
+        rc = T.empty(self.n + 1, dtype=T.int32, device='cuda')
+        nnz = ctypes.c_int()
+        _lib.call('tf_csr_add_count', self.n, ptr(self.row_ptr), ptr(self.col), ptr(other.row_ptr), ptr(other.col),
+                  ptr(cnt), ptr(rc), ctypes.byref(nnz), st)
+        cc = T.empty(max(nnz.value, 1), dtype=T.int32, device='cuda')[:nnz.value]
+        vc = T.empty(max(nnz.value, 1), dtype=T.float64, device='cuda')[:nnz.value]
+        _lib.call('tf_csr_add_fill', self.n, ptr(self.row_ptr), ptr(self.col), ptr(self.val), float(alpha),
+                  ptr(other.row_ptr), ptr(other.col), ptr(other.val), float(beta), ptr(rc), ptr(cc), ptr(vc), st)
+        return DeviceCsrMatrix(rc, cc, vc, self.n)
+
+    def _scaled(self, alpha):
+        T = torch()
+        v = T.empty_like(self.val)
+        _lib.call('tf_scale_f64', self.val.numel(), float(alpha), ptr(self.val), ptr(v), stream())
+        return DeviceCsrMatrix(self.row_ptr, self.col, v, self.n)
+
+    def __add__(self, other):
+        if isinstance(other, DeviceCsrMatrix):
+            return self._combine(1.0, other, 1.0)
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, DeviceCsrMatrix):
+            return self._combine(1.0, other, -1.0)
+        return NotImplemented
+
+    def __mul__(self, alpha):
+        if isinstance(alpha, numbers.Number):
+            return self._scaled(alpha)
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, alpha):
+        if isinstance(alpha, numbers.Number):
+            return self._scaled(1.0 / alpha)
+        return NotImplemented
+
+    def __neg__(self):
+        return self._scaled(-1.0)
