@@ -246,3 +246,32 @@ def test_time_integration_matches_oracle(hip, theta):
     print('theta %.1f: steps %d, Newton its %s / %s, rel err %.2e' % (theta, len(ih), ih, io, err))
     assert th == to and ih == io
     assert err <= 1e-8
+
+
+def test_checkpoint_resume(hip, tmp_path):
+    """State + parameter files (BaseInterface.py:118-215: ``name.npy`` and
+    ``name.params`` JSON) and ``return_step`` (Continuation.py:352-359): stop at
+    Re = 500, save, load into a fresh interface, resume to Re = 1000; the result
+    is the converged state of an uninterrupted run."""
+    import json
+    name = str(tmp_path / 'ldc')
+    itf = hip.Interface({}, 16, 16)
+    cont = hip.Continuation(itf, newton_tolerance=1e-10, printer=None)
+    x0 = cont.newton(itf.vector(), tol=1e-10)
+    x, mu, dx, dmu = cont.continuation(x0, 'Reynolds Number', 0, 500, 100, return_step=True)
+    itf.save_state(name, x)
+    assert numpy.load(name + '.npy').shape == (itf.n,)
+    with open(name + '.params') as f:
+        assert json.load(f)['Reynolds Number'] == pytest.approx(mu)
+    itf2 = hip.Interface({}, 16, 16)
+    x2 = itf2.load_state(name)
+    assert itf2.get_parameter('Reynolds Number') == pytest.approx(mu)
+    assert numpy.array_equal(itf2.array_from_vector(x2), itf.array_from_vector(x))
+    cont2 = hip.Continuation(itf2, newton_tolerance=1e-10, printer=None)
+    xr, mur = cont2.continuation(x2, 'Reynolds Number', mu, 1000, 100, dx=dx, dmu=dmu)
+    itf3 = hip.Interface({}, 16, 16)
+    cont3 = hip.Continuation(itf3, newton_tolerance=1e-10, printer=None)
+    xu, muu = cont3.continuation(cont3.newton(itf3.vector(), tol=1e-10), 'Reynolds Number', 0, 1000, 100)
+    a, b = itf2.array_from_vector(xr), itf3.array_from_vector(xu)
+    assert abs(mur - 1000) <= 1e-3 and abs(muu - 1000) <= 1e-3
+    assert numpy.linalg.norm(a - b) / numpy.linalg.norm(b) <= 1e-6

@@ -316,22 +316,47 @@ __global__ __launch_bounds__(256) void k_sp_rows(int m, const Cpl* __restrict__ 
 }
 
 // O_b[r][c] = (acc ? O_b[r][c] : 0) + sign * sum_q P_b[r][cpl[col c of line_b].idx] * val   (dense x sparse cols)
+// A thread owns column c and keeps its (at most KC) couplings in registers while
+// it sweeps SPC_ROWS rows; consecutive threads write consecutive columns.
+constexpr int SPC_ROWS = 32;
+
+inline dim3 spc_grid(int m, int batch) {
+  return dim3((unsigned)((m + 255) / 256), (unsigned)((m + SPC_ROWS - 1) / SPC_ROWS), (unsigned)batch);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ cpl, const int* __restrict__ ncpl,
                                                  const int* __restrict__ line, T* const* __restrict__ Pp,
                                                  T* const* __restrict__ Op, double sign, int accumulate) {
-  const int rl = blockIdx.x, b = blockIdx.y;
-  const T* P = Pp[b] + (long)rl * m;
-  T* O = Op[b] + (long)rl * m;
-  const long cbase = (long)line[b] * m;
-  for (int c = threadIdx.x; c < m; c += blockDim.x) {
-    const int cnt = ncpl[cbase + c];
-    double acc = 0.0;
-    for (int q = 0; q < cnt; ++q) {
-      const Cpl cp = cpl[(cbase + c) * KC + q];
-      acc += (double)P[cp.idx] * cp.val;
+  const int b = blockIdx.z;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= m) return;
+  const int r0 = blockIdx.y * SPC_ROWS;
+  const int r1 = r0 + SPC_ROWS < m ? r0 + SPC_ROWS : m;
+  const T* P = Pp[b];
+  T* O = Op[b];
+  const long col = (long)line[b] * m + c;
+  const int cnt = ncpl[col];
+  int idx[KC];
+  double val[KC];
+#pragma unroll
+  for (int q = 0; q < KC; ++q) {
+    idx[q] = 0;
+    val[q] = 0.0;
+    if (q < cnt) {
+      const Cpl cp = cpl[col * KC + q];
+      idx[q] = cp.idx;
+      val[q] = cp.val;
     }
-    O[c] = (T)((accumulate ? (double)O[c] : 0.0) + sign * acc);
+  }
+  for (int r = r0; r < r1; ++r) {
+    const T* prow = P + (long)r * m;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+      if (q < cnt) acc += (double)prow[idx[q]] * val[q];
+    T* o = O + (long)r * m + c;
+    *o = (T)((accumulate ? (double)*o : 0.0) + sign * acc);
   }
 }
 
@@ -645,17 +670,21 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
       // D_k -= P1 U_e1 (rows of line k-1 into the columns of line k: Ut)
       // D_k -= P2 L_e2 (rows of line k+1 into the columns of line k: Lt)
       if (L.n_p1)
-        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_p1), dim3(256), 0, st, m, h->Ut, h->nUt, L.k_hase1_line,
+        hipLaunchKernelGGL(k_sp_cols<T>, spc_grid(m, L.n_p1), dim3(256), 0, st, m, h->Ut, h->nUt,
+                           L.k_hase1_line,
                            (T* const*)L.g_p1_C, (T* const*)L.g_d1_A, -1.0, 1);
       if (L.n_p2)
-        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_p2), dim3(256), 0, st, m, h->Lt, h->nLt, L.k_hase2_line,
+        hipLaunchKernelGGL(k_sp_cols<T>, spc_grid(m, L.n_p2), dim3(256), 0, st, m, h->Lt, h->nLt,
+                           L.k_hase2_line,
                            (T* const*)L.g_p2_C, (T* const*)L.g_d2_A, -1.0, 1);
       // L_k = -P1 L_e1 (columns of line k-2: Lt) ; U_k = -P2 U_e2 (columns of line k+2: Ut)
       if (L.n_fa)
-        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_fa), dim3(256), 0, st, m, h->Lt, h->nLt, L.k_fa_line,
+        hipLaunchKernelGGL(k_sp_cols<T>, spc_grid(m, L.n_fa), dim3(256), 0, st, m, h->Lt, h->nLt,
+                           L.k_fa_line,
                            (T* const*)L.g_fa_A, (T* const*)L.g_fa_C, -1.0, 0);
       if (L.n_fb)
-        hipLaunchKernelGGL(k_sp_cols<T>, dim3(m, L.n_fb), dim3(256), 0, st, m, h->Ut, h->nUt, L.k_fb_line,
+        hipLaunchKernelGGL(k_sp_cols<T>, spc_grid(m, L.n_fb), dim3(256), 0, st, m, h->Ut, h->nUt,
+                           L.k_fb_line,
                            (T* const*)L.g_fb_A, (T* const*)L.g_fb_C, -1.0, 0);
     } else {
       gemm<T>(m, m, m, L.g_p1_A, m, L.g_p1_B, m, L.g_p1_C, m, 1.0, 0.0, L.n_p1, st);
