@@ -127,13 +127,27 @@ __global__ __launch_bounds__(64 * DOF) void k_assemble(Grid g, Physics ph, const
     block_inclusive_scan<NT>(s, t);
     if (t == NT - 1) block_sums[blockIdx.x] = s[NT - 1];
   } else {
+    // The workgroup's rows are contiguous in the CSR arrays: when its entries fit,
+    // the rows are emitted into LDS and the segment is written out coalesced.
+    constexpr int CAP = NT * 12;
+    __shared__ int s_col[CAP];
+    __shared__ double s_val[CAP];
     s[lr] = active ? counts[row] : 0;
     __syncthreads();
     block_inclusive_scan<NT>(s, t);
-    const int off = block_off[blockIdx.x] + (lr > 0 ? s[lr - 1] : 0);
-    if (active) {
-      row_ptr[row] = off;
-      dispatch_row<DIM, DOF, BUOY, MODE>(d1, c, ph, row, nullptr, col, val, off);
+    const int base = block_off[blockIdx.x];
+    const int total = s[NT - 1];
+    const int loc = lr > 0 ? s[lr - 1] : 0;
+    if (active) row_ptr[row] = base + loc;
+    if (total <= CAP) {
+      if (active) dispatch_row<DIM, DOF, BUOY, MODE>(d1, c, ph, row, nullptr, s_col, s_val, loc);
+      __syncthreads();
+      for (int e = t; e < total; e += NT) {
+        col[base + e] = s_col[e];
+        val[base + e] = s_val[e];
+      }
+    } else if (active) {
+      dispatch_row<DIM, DOF, BUOY, MODE>(d1, c, ph, row, nullptr, col, val, base + loc);
     }
   }
 }

@@ -43,11 +43,10 @@ struct Mfma<float> {
   }
 };
 
-template <typename T, int BM, int BN, int MINW = 1>
+template <typename T, int BM, int BN, int MINW = 1, int BK = 16>
 __global__ __launch_bounds__(256, MINW) void k_gemm_mfma(int M, int N, int K, T* const* __restrict__ Ap, int lda,
                                                    T* const* __restrict__ Bp, int ldb, T* const* __restrict__ Cp,
                                                    int ldc, double alpha, double beta) {
-  constexpr int BK = 16;
   constexpr int WM = BM / 2, WN = BN / 2;      // wave sub-tile
   constexpr int TM = WM / 16, TN = WN / 16;    // MFMA tiles per wave
   constexpr int PA = BM + 4, PB_ = BN + 4;     // padded LDS rows
@@ -75,7 +74,7 @@ __global__ __launch_bounds__(256, MINW) void k_gemm_mfma(int M, int N, int K, T*
 #pragma unroll
     for (int q = 0; q < LA; ++q) {
       const int e = t + q * 256;
-      const int r = e / BK, kk = e % BK;  // 16 consecutive threads read one row's 16 k
+      const int r = e / BK, kk = e % BK;  // BK consecutive threads read one row's BK k
       const int gm = m0 + r, gk = k0 + kk;
       ra[q] = (gm < M && gk < K) ? A[(long)gm * lda + gk] : (T)0;
     }
