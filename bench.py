@@ -68,10 +68,25 @@ def workload(name, n, target, ds):
                      'Re 0->%g, ds=%g, Newton tol 1e-8' % (n, n, 3 * n * n, target, ds)}
 
 
-def run_continuation(tf, torch, spec, precision, maxit, restart=30):
+# solver variants: the plain FP64 method, mixed precision with the FP32 factor
+# applied from its FP32 blocks, and from FP16 copies of them
+VARIANTS = {
+    'fp64': {'Precision': 'fp64'},
+    'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
+    'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
+}
+DTYPES = {
+    'fp64': 'f64',
+    'mixed': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
+    'mixed16': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies '
+               'of the factor blocks with f32 row scales)',
+}
+
+
+def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     import numpy
     params = dict(spec['params'])
-    params['HIP Solver'] = {'Precision': precision, 'Restart': restart}
+    params['HIP Solver'] = dict(VARIANTS[variant], Restart=restart)
     itf = tf.Interface(params, *spec['shape'])
     cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
     start = itf.vector()
@@ -262,7 +277,7 @@ def main():
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--target', type=float, default=2000.0)
     ap.add_argument('--ds', type=float, default=100.0)
-    ap.add_argument('--variants', default='fp64,mixed')
+    ap.add_argument('--variants', default='fp64,mixed,mixed16')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch', 'rb', 'ldc3d'],
                     help="'ldc': one 256x256 cavity continuation (default); 'batch': 64 seeded cavity "
@@ -321,7 +336,7 @@ def main():
     pr, kt = profiled_run(tf, torch, spec, head, maxit)
     pid, (calls, ms, work) = max(kt.items(), key=lambda kv: kv[1][1])
     name, bound, wunit = PROFILED[pid]
-    single = head == 'mixed'
+    single = head != 'fp64'
     if bound == 'mfma':
         peak = FP32_PEAK_TFLOPS if single else FP64_PEAK_TFLOPS
         achieved = work / (ms * 1e-3) / 1e12
@@ -335,11 +350,12 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        key = '%s %s' % (name, 'f32' if single else 'f64')
+        key = '%s %s' % (name, {'fp64': 'f64', 'mixed': 'f32', 'mixed16': 'f16'}[head])
         if key in tj:
             traffic = tj[key].get('bytes_per_launch')
     roof = {'bound': bound, 'achieved': achieved, 'peak': peak, 'unit': unit, 'frac': achieved / peak,
-            'traffic': traffic, 'kernel': '%s (%s)' % (name, 'FP32' if single else 'FP64'),
+            'traffic': traffic,
+            'kernel': '%s (%s)' % (name, {'fp64': 'FP64', 'mixed': 'FP32', 'mixed16': 'FP32, FP16 blocks'}[head]),
             'launches': calls, 'avg_launch_us': ms / max(calls, 1) * 1e3,
             'per_launch_work': work / max(calls, 1), 'work_unit': wunit,
             'profiled_run_steps_per_s': pr['steps_per_s'],
@@ -363,7 +379,7 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64' if head == 'fp64' else 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
+        'dtype': DTYPES[head],
         'data': 'synthetic',
         'config': {
             'workload': spec['label'],

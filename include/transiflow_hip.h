@@ -170,6 +170,12 @@ int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
  * one launch plus two staging copies (no reference counterpart; launch overhead on small
  * grids and batches). */
 int tf_bcr_set_graph(tf_bcr* h, int on);
+/* on != 0 (single != 0 only): the FP32 apply (tf_bcr_solve_f32) reads FP16 copies of the
+ * factor blocks with one power-of-two FP32 scale per row, remade after every factorisation:
+ * half the bytes of the HBM-bound apply for the inner solves of the mixed-precision solver
+ * (no reference counterpart; the reference's splu factor is FP64).  The first call
+ * allocates the copies; an existing factor is converted on ``stream``. */
+int tf_bcr_set_half(tf_bcr* h, int on, void* stream);
 
 /* ---- learned preconditioner: convolutions on the cell grid (no reference counterpart) ----
  * Activations cell-major [ncell][C] in BF16 (uint16 storage); layer weights in the

@@ -22,9 +22,12 @@
 // 0 the couplings are the sparse CSR entries (a few per row), so the level-0
 // products are sparse-dense and cost O(m^2).  The factor (Y, P1, P2, L, U) is
 // kept in the storage type T (double, or float for the mixed-precision
-// variant, whose factorisation also runs in FP32).
+// variant, whose factorisation also runs in FP32).  For the FP32 apply of that
+// variant the blocks can also be copied to FP16 with one FP32 scale per row
+// (tf_bcr_set_half), which halves the bytes the apply reads.
 //
-// Solve: forward  r_k -= P1 r_e1 + P2 r_e2  (level by level),
+// Solve: forward  r_k -= P1 r_e1 + P2 r_e2  (level by level; at level 0, where
+//                 L_k and U_k are sparse, as r_k -= L_k (Y_e1 r_e1) + U_k (Y_e2 r_e2)),
 //        backward x_e = Y_e (r_e - L_e x_a - U_e x_b)  (levels in reverse).
 //
 // Replaces the sparse LU of the reference's direct solve
@@ -364,7 +367,14 @@ __global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ 
 //
 // Every solve kernel is a batch of row-times-vector products over m-long rows.
 // A group of lanes owns one row and reads it with 16-byte loads when the row
-// length and the vectors allow it (see RowGroup for the group size).
+// length and the vectors allow it (see RowGroup for the group size).  The rows
+// come from the factor's own blocks (Blocks<T>), or from FP16 copies of the
+// FP32 blocks with one power-of-two FP32 scale per row (HalfBlocks): half the
+// bytes for the FP32 apply of the mixed-precision solver.
+
+typedef _Float16 f16;
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct Vec16;
@@ -379,18 +389,19 @@ struct Vec16<double> {
   static constexpr int W = 2;
 };
 
-// 16 lanes per row (measured: 8-lane groups for FP32 rows are slower).
-template <typename T>
+// Lanes per row: 16 for FP32/FP64 rows (measured: 8-lane groups for FP32 rows
+// are slower); the FP16 copies use GroupOf<HalfBlocks>.
+template <int G_>
 struct RowGroup {
-  static constexpr int G = 16;
+  static constexpr int G = G_;
   static constexpr int ROWS = 256 / G;
 };
+constexpr int HALF_G = 16;
 
 // sum over c of a[c] x[c] by the G-lane group g (the result is in every lane of the group)
-template <typename T, typename V>
+template <int G, typename T, typename V>
 __device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g,
                                             bool vec) {
-  constexpr int G = RowGroup<T>::G;
   double acc = 0.0;
   if constexpr (sizeof(T) == sizeof(V)) {
     if (vec) {
@@ -414,21 +425,73 @@ __device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __
   return acc;
 }
 
+// FP16 row against an FP32 vector: 8 halves (16 bytes) and two float4 per lane per step
+template <int G>
+__device__ __forceinline__ double group_dot(const f16* __restrict__ a, const float* __restrict__ x, int m, int g,
+                                            bool vec) {
+  double acc = 0.0;
+  if (vec) {
+    const f16x8* av = (const f16x8*)a;
+    const f32x4* xv = (const f32x4*)x;
+#pragma unroll 2
+    for (int c = g; c < m / 8; c += G) {
+      const f16x8 p = av[c];
+      const f32x4 q0 = xv[2 * c], q1 = xv[2 * c + 1];
+      float s = 0.0f;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) s = fmaf((float)p[w], q0[w], s);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) s = fmaf((float)p[4 + w], q1[w], s);
+      acc += (double)s;
+    }
+  } else {
+    for (int c = g; c < m; c += G) acc += (double)(float)a[c] * (double)x[c];
+  }
+  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+  return acc;
+}
+
+// Rows of a batch of m x m blocks in the factor's storage type.
+template <typename T>
+struct Blocks {
+  T* const* p;
+  __device__ __forceinline__ const T* row(int b, int r, int m) const { return p[b] + (long)r * m; }
+  __device__ __forceinline__ double scale(int, int) const { return 1.0; }
+};
+
+// FP16 copies: row r of block b is s[b][r] * p[b][r m ... r m + m).
+struct HalfBlocks {
+  f16* const* p;
+  float* const* s;
+  __device__ __forceinline__ const f16* row(int b, int r, int m) const { return p[b] + (long)r * m; }
+  __device__ __forceinline__ double scale(int b, int r) const { return (double)s[b][r]; }
+};
+
+template <typename B>
+struct GroupOf : RowGroup<16> {};
+template <>
+struct GroupOf<HalfBlocks> : RowGroup<HALF_G> {};
+
+template <typename B, typename V>
+__device__ __forceinline__ double row_dot(const B& blk, int b, int r, const V* __restrict__ x, int m, int g,
+                                          bool vec) {
+  return blk.scale(b, r) * group_dot<GroupOf<B>::G>(blk.row(b, r, m), x, m, g, vec);
+}
+
 // rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines
-template <typename T, typename V>
+template <typename B, typename V>
 __global__ __launch_bounds__(256) void k_fwd(int m, int vec, const int* __restrict__ k_line,
-                                             const int* __restrict__ e1, const int* __restrict__ e2,
-                                             T* const* __restrict__ P1p, T* const* __restrict__ P2p,
+                                             const int* __restrict__ e1, const int* __restrict__ e2, B P1, B P2,
                                              V* __restrict__ rw) {
-  constexpr int G = RowGroup<T>::G;
+  constexpr int G = GroupOf<B>::G;
   const int b = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   double acc = 0.0;
-  if (e1[b] >= 0) acc += group_dot(P1p[b] + (long)rr * m, rw + (long)e1[b] * m, m, g, vec);
-  if (e2[b] >= 0) acc += group_dot(P2p[b] + (long)rr * m, rw + (long)e2[b] * m, m, g, vec);
+  if (e1[b] >= 0) acc += row_dot(P1, b, rr, rw + (long)e1[b] * m, m, g, vec);
+  if (e2[b] >= 0) acc += row_dot(P2, b, rr, rw + (long)e2[b] * m, m, g, vec);
   if (live && g == 0) {
     V* out = rw + (long)k_line[b] * m + r;
     *out = (V)((double)*out - acc);
@@ -436,18 +499,17 @@ __global__ __launch_bounds__(256) void k_fwd(int m, int vec, const int* __restri
 }
 
 // t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, or the level-0 sparse rows)
-template <typename T, typename V>
+template <typename B, typename V>
 __global__ __launch_bounds__(256) void k_bwd_t(int m, int vec, int sparse, const int* __restrict__ e_line,
                                                const int* __restrict__ a_line, const int* __restrict__ b_line,
-                                               T* const* __restrict__ Lp, T* const* __restrict__ Up,
-                                               const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
+                                               B Lb, B Ub, const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
                                                const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
                                                const V* __restrict__ rw, const V* __restrict__ x,
                                                V* __restrict__ tbuf) {
-  constexpr int G = RowGroup<T>::G;
+  constexpr int G = GroupOf<B>::G;
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long row = (long)e_line[bi] * m + rr;
@@ -461,26 +523,69 @@ __global__ __launch_bounds__(256) void k_bwd_t(int m, int vec, int sparse, const
         for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
     }
   } else {
-    if (a >= 0) acc += group_dot(Lp[bi] + (long)rr * m, x + (long)a * m, m, g, vec);
-    if (bb >= 0) acc += group_dot(Up[bi] + (long)rr * m, x + (long)bb * m, m, g, vec);
+    if (a >= 0) acc += row_dot(Lb, bi, rr, x + (long)a * m, m, g, vec);
+    if (bb >= 0) acc += row_dot(Ub, bi, rr, x + (long)bb * m, m, g, vec);
   }
   if (live && g == 0) tbuf[row] = (V)((double)rw[row] - acc);
 }
 
 // x[e] = Y_e t[e]
-template <typename T, typename V>
-__global__ __launch_bounds__(256) void k_bwd_x(int m, int vec, const int* __restrict__ e_line,
-                                               T* const* __restrict__ Yp, const V* __restrict__ tbuf,
-                                               V* __restrict__ x) {
-  constexpr int G = RowGroup<T>::G;
+template <typename B, typename V>
+__global__ __launch_bounds__(256) void k_bwd_x(int m, int vec, const int* __restrict__ e_line, B Yb,
+                                               const V* __restrict__ tbuf, V* __restrict__ x) {
+  constexpr int G = GroupOf<B>::G;
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * RowGroup<T>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
-  const double acc = group_dot(Yp[bi] + (long)rr * m, tbuf + base, m, g, vec);
+  const double acc = row_dot(Yb, bi, rr, tbuf + base, m, g, vec);
   if (live && g == 0) x[base + r] = (V)acc;
+}
+
+// Level 0, kept lines: rw[k] -= L_k y[e1] + U_k y[e2] with the sparse couplings and
+// y[e] = Y_e rw[e] (so P1 rw[e1] = L_k Y_e1 rw[e1] without reading P1).  Thread per row.
+template <typename V>
+__global__ __launch_bounds__(256) void k_fwd_sparse(int m, const int* __restrict__ k_line,
+                                                    const int* __restrict__ e1, const int* __restrict__ e2,
+                                                    const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
+                                                    const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
+                                                    const V* __restrict__ y, V* __restrict__ rw) {
+  const int b = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const long row = (long)k_line[b] * m + r;
+  const int a = e1[b], c = e2[b];
+  double acc = 0.0;
+  if (a >= 0)
+    for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)y[(long)a * m + Lr[row * KC + q].idx];
+  if (c >= 0)
+    for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)y[(long)c * m + Ur[row * KC + q].idx];
+  rw[row] = (V)((double)rw[row] - acc);
+}
+
+// FP16 copy of a batch of m x m FP32 blocks with one power-of-two scale per row:
+// the row maximum maps below 2^15 (no overflow), the scaling itself is exact, and
+// only entries more than 2^-29 below the row maximum fall into FP16 subnormals.
+// One wave per row; grid (ceil(m / 4), batch).
+__global__ __launch_bounds__(256) void k_to_half(int m, float* const* __restrict__ src, f16* const* __restrict__ dst,
+                                                 float* const* __restrict__ scl) {
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= m) return;  // the whole wave leaves together
+  const float* a = src[b] + (long)r * m;
+  f16* o = dst[b] + (long)r * m;
+  float mx = 0.0f;
+  for (int c = lane; c < m; c += 64) mx = fmaxf(mx, fabsf(a[c]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  int e = 0;
+  if (mx > 0.0f && mx <= 3.0e38f) (void)frexpf(mx, &e);  // mx < 2^e
+  if (e < -100) e = -100;
+  const float inv = ldexpf(1.0f, 15 - e);
+  for (int c = lane; c < m; c += 64) o[c] = (f16)(a[c] * inv);
+  if (lane == 0) scl[b][r] = ldexpf(1.0f, e - 15);
 }
 
 template <typename V>
@@ -558,6 +663,9 @@ struct BcrLevelDev {
   void **g_fa_A, **g_fa_B, **g_fa_C;  // L_k = -P1 L_e1
   void **g_fb_A, **g_fb_B, **g_fb_C;  // U_k = -P2 U_e2
   void **z_fa, **z_fb;                // L_k (U_k) that become zero
+  // FP16 copies read by the FP32 apply (tf_bcr_set_half): blocks and row scales
+  void **Yh, **Lh, **Uh, **P1h, **P2h;
+  void **Ys, **Ls, **Us, **P1s, **P2s;
 };
 
 struct BcrGraph {
@@ -591,10 +699,21 @@ struct tf_bcr {
   int* host_flags;
   void* rw;    // n (solve workspace)
   void* tb;    // n
+  void* yb;    // n (level-0 Y_e r_e)
   void* pool;  // device pointer/index pool
   std::vector<LevelPlan> plan;
   std::vector<BcrLevelDev> lev;
   size_t solve_bytes;  // algorithmic bytes of one solve (dense blocks read once, vectors)
+  // FP16 copies of the blocks the apply reads (single-precision factors only)
+  int half;            // the FP32 apply reads the FP16 copies
+  int factored;        // a factorisation has run (the copies can be made)
+  void* H;             // FP16 blocks: D, L, U (N m^2 each), P1, P2 (slots m^2 each)
+  void* S;             // FP32 row scales, same layout with m per block
+  void* hpool;         // per-level FP16 pointer lists + the conversion list
+  void **cv_src, **cv_dst, **cv_scl;  // blocks converted after each factorisation
+  int ncv;
+  size_t slots;
+  size_t solve_bytes_half;
 };
 
 namespace {
@@ -701,8 +820,68 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
     if (L.n_nofb)
       hipLaunchKernelGGL(k_zero_many<T>, dim3(256, L.n_nofb), dim3(256), 0, st, mm, (T* const*)L.z_fb);
   }
+  h->factored = 1;
+  if constexpr (sizeof(T) == 4) {
+    if (h->half && h->ncv > 0)
+      hipLaunchKernelGGL(k_to_half, dim3((m + 3) / 4, h->ncv), dim3(256), 0, st, m, (float* const*)h->cv_src,
+                         (f16* const*)h->cv_dst, (float* const*)h->cv_scl);
+  }
   TF_LAUNCH_CHECK("bcr factor");
   return 0;
+}
+
+// forward and backward sweeps over the levels, rows read through the block source B
+template <typename B, typename V>
+void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&), B Lb(const BcrLevelDev&),
+           B Ub(const BcrLevelDev&), B Yb(const BcrLevelDev&), V* rw, V* tb, V* xo, hipStream_t st) {
+  const int m = h->m;
+  const dim3 blk(256);
+  const unsigned gx = (unsigned)((m + GroupOf<B>::ROWS - 1) / GroupOf<B>::ROWS);
+  V* yb = (V*)h->yb;
+  for (size_t l = 0; l < h->lev.size(); ++l) {
+    const BcrLevelDev& L = h->lev[l];
+    if (!L.nk) continue;
+    if (l == 0) {
+      // the level-0 couplings are sparse: y_e = Y_e r_e, then r_k -= L_k y_e1 + U_k y_e2
+      // (one Y block per eliminated line instead of two multipliers per kept line)
+      hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, Yb(L), (const V*)rw, yb);
+      hipLaunchKernelGGL((k_fwd_sparse<V>), dim3((m + 255) / 256, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr,
+                         h->nLr, h->Ur, h->nUr, (const V*)yb, rw);
+    } else {
+      hipLaunchKernelGGL((k_fwd<B, V>), dim3(gx, L.nk), blk, 0, st, m, vec, L.k, L.k1, L.k2, P1(L), P2(L), rw);
+    }
+  }
+  for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
+    const BcrLevelDev& L = h->lev[l];
+    // level 0 uses the sparse couplings (no dense L/U rows are read)
+    hipLaunchKernelGGL((k_bwd_t<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, l == 0 ? 1 : 0, L.e, L.ea, L.eb, Lb(L),
+                       Ub(L), h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)xo, tb);
+    hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, Yb(L), (const V*)tb, xo);
+  }
+}
+
+template <typename T, typename V>
+void levels(tf_bcr* h, V* rw, V* tb, V* xo, hipStream_t st) {
+  // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
+  constexpr int W = 16 / (int)sizeof(T);
+  const int vec = (sizeof(T) == sizeof(V) && h->m % W == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
+  typedef Blocks<T> B;
+  sweep<B, V>(
+      h, vec, [](const BcrLevelDev& L) { return B{(T* const*)L.P1}; },
+      [](const BcrLevelDev& L) { return B{(T* const*)L.P2}; }, [](const BcrLevelDev& L) { return B{(T* const*)L.Le}; },
+      [](const BcrLevelDev& L) { return B{(T* const*)L.Ue}; }, [](const BcrLevelDev& L) { return B{(T* const*)L.Ye}; },
+      rw, tb, xo, st);
+}
+
+void levels_half(tf_bcr* h, float* rw, float* tb, float* xo, hipStream_t st) {
+  const int vec = (h->m % 8 == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
+  typedef HalfBlocks B;
+  sweep<B, float>(
+      h, vec, [](const BcrLevelDev& L) { return B{(f16* const*)L.P1h, (float* const*)L.P1s}; },
+      [](const BcrLevelDev& L) { return B{(f16* const*)L.P2h, (float* const*)L.P2s}; },
+      [](const BcrLevelDev& L) { return B{(f16* const*)L.Lh, (float* const*)L.Ls}; },
+      [](const BcrLevelDev& L) { return B{(f16* const*)L.Uh, (float* const*)L.Us}; },
+      [](const BcrLevelDev& L) { return B{(f16* const*)L.Yh, (float* const*)L.Ys}; }, rw, tb, xo, st);
 }
 
 template <typename T, typename V>
@@ -710,7 +889,8 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
   const int m = h->m;
   V* rw = (V*)h->rw;
   V* tb = (V*)h->tb;
-  TF_PROF_WORK(tf::PROF_SOLVE, (double)h->solve_bytes);
+  TF_PROF_WORK(tf::PROF_SOLVE, (double)((sizeof(T) == 4 && sizeof(V) == 4 && h->half) ? h->solve_bytes_half
+                                                                                     : h->solve_bytes));
   TF_PROF_BEGIN(tf::PROF_SOLVE, st);
   const bool nat = !h->order.ylines;
   V* xo = nat ? x : (V*)h->xw;   // the kernels work in line-major order
@@ -718,24 +898,12 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st) {
     hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
   else
     hipLaunchKernelGGL(k_to_lines<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, r, rw);
-  const dim3 blk(256);
-  // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
-  constexpr int W = 16 / (int)sizeof(T);
-  const int vec = (sizeof(T) == sizeof(V) && m % W == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
-  const unsigned gx = (unsigned)((m + RowGroup<T>::ROWS - 1) / RowGroup<T>::ROWS);
-  for (size_t l = 0; l < h->lev.size(); ++l) {
-    BcrLevelDev& L = h->lev[l];
-    if (L.nk)
-      hipLaunchKernelGGL((k_fwd<T, V>), dim3(gx, L.nk), blk, 0, st, m, vec, L.k, L.k1, L.k2, (T* const*)L.P1,
-                         (T* const*)L.P2, rw);
-  }
-  for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
-    BcrLevelDev& L = h->lev[l];
-    hipLaunchKernelGGL((k_bwd_t<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, l == 0 ? 1 : 0, L.e, L.ea, L.eb,
-                       (T* const*)L.Le, (T* const*)L.Ue, h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)xo, tb);
-    hipLaunchKernelGGL((k_bwd_x<T, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, (T* const*)L.Ye, (const V*)tb,
-                       xo);
-  }
+  bool half = false;
+  if constexpr (sizeof(T) == 4 && sizeof(V) == 4) half = h->half != 0;
+  if (half)
+    levels_half(h, (float*)rw, (float*)tb, (float*)xo, st);
+  else
+    levels<T, V>(h, rw, tb, xo, st);
   if (!nat) hipLaunchKernelGGL(k_to_natural<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, (const V*)xo, x);
   TF_PROF_END(tf::PROF_SOLVE, st);
   TF_LAUNCH_CHECK("bcr solve");
@@ -835,6 +1003,7 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
   size_t slots = 0;
   for (const LevelPlan& L : h->plan) slots += L.k.size();
   if (slots < 1) slots = 1;
+  h->slots = slots;
   TF_HIP(hipMalloc(&h->P1, h->es * mm * slots));
   TF_HIP(hipMalloc(&h->P2, h->es * mm * slots));
   TF_HIP(hipMalloc(&h->Pinv, h->es * PB * PB * maxb));
@@ -847,6 +1016,7 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
   TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
   TF_HIP(hipMalloc(&h->rw, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->tb, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->yb, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->xw, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->gin, sizeof(double) * (size_t)n));
   TF_HIP(hipMalloc(&h->gout, sizeof(double) * (size_t)n));
@@ -965,17 +1135,23 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
     offs.push_back(o);
   }
   // one solve reads every Y once, the dense L/U of the lines eliminated at
-  // levels >= 1, the multipliers of every kept line, and moves ~6 vectors
+  // levels >= 1, the multipliers of the kept lines at levels >= 1, the level-0
+  // Y blocks a second time (forward), and moves ~6 vectors
   {
     size_t blocks = 0;
     for (size_t l = 0; l < h->plan.size(); ++l) {
       const LevelPlan& L = h->plan[l];
       blocks += L.e.size();
-      if (l > 0)
+      if (l > 0) {
         for (size_t b = 0; b < L.e.size(); ++b) blocks += (L.ea[b] >= 0) + (L.eb[b] >= 0);
-      for (size_t b = 0; b < L.k.size(); ++b) blocks += (L.k1[b] >= 0) + (L.k2[b] >= 0);
+        for (size_t b = 0; b < L.k.size(); ++b) blocks += (L.k1[b] >= 0) + (L.k2[b] >= 0);
+      } else if (!L.k.empty()) {
+        blocks += L.e.size();
+      }
     }
     h->solve_bytes = blocks * bm + 6 * sizeof(double) * (size_t)n;
+    // FP16 copies: 2 bytes per entry + one FP32 scale per row; FP32 vectors
+    h->solve_bytes_half = blocks * (2 * mm + 4 * (size_t)m) + 6 * sizeof(float) * (size_t)n;
   }
   pb.align();
   TF_HIP(hipMalloc(&h->pool, pb.bytes.size() + 64));
@@ -1041,7 +1217,7 @@ int tf_bcr_destroy(tf_bcr* h) {
   (void)hipFree(h->gout);
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
-                  (void*)h->flags, h->rw, h->tb, h->xw, h->pool})
+                  (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;
@@ -1072,6 +1248,120 @@ int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream) {
 int tf_bcr_set_graph(tf_bcr* h, int on) {
   if (!h) return tf::fail("null handle");
   h->use_graph = on ? 1 : 0;
+  return 0;
+}
+
+// on != 0 (single-precision factors only): the FP32 apply reads FP16 copies of
+// the factor blocks, one power-of-two FP32 scale per row, remade after every
+// factorisation -- half the bytes of the FP32 blocks.  The first call allocates
+// the copies; a factor that already exists is converted on ``stream``.
+int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
+  if (!h) return tf::fail("null handle");
+  if (on && !h->single) return tf::fail("FP16 block copies need a single-precision factor");
+  hipStream_t st = (hipStream_t)stream;
+  const int m = h->m;
+  const size_t mm = (size_t)m * m, N = h->N, slots = h->slots;
+  if (on && !h->H) {
+    TF_HIP(hipMalloc(&h->H, sizeof(f16) * mm * (3 * N + 2 * slots)));
+    TF_HIP(hipMalloc(&h->S, sizeof(float) * (size_t)m * (3 * N + 2 * slots)));
+    f16* H = (f16*)h->H;
+    float* S = (float*)h->S;
+    // block j of buffer q (0 D, 1 L, 2 U, 3 P1, 4 P2) in the copies and in the FP32 factor
+    auto hb = [&](int q, int j) -> void* {
+      const size_t base = q < 3 ? q * N : 3 * N + (q - 3) * slots;
+      return (void*)(H + (base + j) * mm);
+    };
+    auto sb = [&](int q, int j) -> void* {
+      const size_t base = q < 3 ? q * N : 3 * N + (q - 3) * slots;
+      return (void*)(S + (base + j) * (size_t)m);
+    };
+    char* src[5] = {(char*)h->D, (char*)h->Lm, (char*)h->Um, (char*)h->P1, (char*)h->P2};
+    auto fb = [&](int q, int j) -> void* { return (void*)(src[q] + sizeof(float) * mm * (size_t)j); };
+    PoolBuilder pb;
+    struct HOff {
+      size_t Yh, Lh, Uh, P1h, P2h, Ys, Ls, Us, P1s, P2s;
+    };
+    std::vector<HOff> offs;
+    std::vector<void*> cs, cd, cc;
+    auto conv = [&](int q, int j) {
+      cs.push_back(fb(q, j));
+      cd.push_back(hb(q, j));
+      cc.push_back(sb(q, j));
+    };
+    for (size_t l = 0; l < h->plan.size(); ++l) {
+      const LevelPlan& L = h->plan[l];
+      std::vector<void*> yh, lh, uh, ys, ls, us, p1h, p2h, p1s, p2s;
+      for (size_t b = 0; b < L.e.size(); ++b) {
+        const int e = L.e[b];
+        yh.push_back(hb(0, e));
+        ys.push_back(sb(0, e));
+        lh.push_back(hb(1, e));
+        ls.push_back(sb(1, e));
+        uh.push_back(hb(2, e));
+        us.push_back(sb(2, e));
+        conv(0, e);
+        if (l > 0 && L.ea[b] >= 0) conv(1, e);
+        if (l > 0 && L.eb[b] >= 0) conv(2, e);
+      }
+      for (size_t b = 0; b < L.k.size(); ++b) {
+        const int sl = L.kslot[b];
+        p1h.push_back(hb(3, sl));
+        p1s.push_back(sb(3, sl));
+        p2h.push_back(hb(4, sl));
+        p2s.push_back(sb(4, sl));
+        // (level 0 applies its multipliers through Y and the sparse couplings)
+        if (l > 0 && L.k1[b] >= 0) conv(3, sl);
+        if (l > 0 && L.k2[b] >= 0) conv(4, sl);
+      }
+      HOff o;
+      o.Yh = pb.add_ptrs(yh);
+      o.Lh = pb.add_ptrs(lh);
+      o.Uh = pb.add_ptrs(uh);
+      o.Ys = pb.add_ptrs(ys);
+      o.Ls = pb.add_ptrs(ls);
+      o.Us = pb.add_ptrs(us);
+      o.P1h = pb.add_ptrs(p1h);
+      o.P2h = pb.add_ptrs(p2h);
+      o.P1s = pb.add_ptrs(p1s);
+      o.P2s = pb.add_ptrs(p2s);
+      offs.push_back(o);
+    }
+    const size_t ocs = pb.add_ptrs(cs), ocd = pb.add_ptrs(cd), occ = pb.add_ptrs(cc);
+    pb.align();
+    TF_HIP(hipMalloc(&h->hpool, pb.bytes.size() + 64));
+    TF_HIP(hipMemcpy(h->hpool, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice));
+    char* P = (char*)h->hpool;
+    for (size_t l = 0; l < offs.size(); ++l) {
+      BcrLevelDev& L = h->lev[l];
+      const HOff& o = offs[l];
+      L.Yh = (void**)(P + o.Yh);
+      L.Lh = (void**)(P + o.Lh);
+      L.Uh = (void**)(P + o.Uh);
+      L.Ys = (void**)(P + o.Ys);
+      L.Ls = (void**)(P + o.Ls);
+      L.Us = (void**)(P + o.Us);
+      L.P1h = (void**)(P + o.P1h);
+      L.P2h = (void**)(P + o.P2h);
+      L.P1s = (void**)(P + o.P1s);
+      L.P2s = (void**)(P + o.P2s);
+    }
+    h->cv_src = (void**)(P + ocs);
+    h->cv_dst = (void**)(P + ocd);
+    h->cv_scl = (void**)(P + occ);
+    h->ncv = (int)cs.size();
+  }
+  if ((on != 0) != (h->half != 0) && h->g32.exec) {
+    // the captured FP32 apply reads the other storage: capture it again
+    (void)hipGraphExecDestroy(h->g32.exec);
+    (void)hipGraphDestroy(h->g32.graph);
+    h->g32 = BcrGraph{};
+  }
+  h->half = on ? 1 : 0;
+  if (h->half && h->factored && h->ncv > 0) {
+    hipLaunchKernelGGL(k_to_half, dim3((m + 3) / 4, h->ncv), dim3(256), 0, st, m, (float* const*)h->cv_src,
+                       (f16* const*)h->cv_dst, (float* const*)h->cv_scl);
+    TF_LAUNCH_CHECK("bcr FP16 copies");
+  }
   return 0;
 }
 

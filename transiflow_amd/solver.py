@@ -26,6 +26,11 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
 ``'Apply Graph'``      replay the line-factor apply as one captured hipGraph
                        (True);
+``'Apply Storage'``    (mixed, line factor) ``'fp16'`` (default) or ``'fp32'``:
+                       with ``'fp16'`` the inner FP32 apply reads FP16 copies of
+                       the factor blocks (one FP32 scale per row), half the bytes
+                       of the HBM-bound apply; the FP64 refinement keeps the
+                       result to the same tolerance;
 ``'Learned Network'``  (mixed) a ``learned.ConvNet`` (or the path of its saved
                        weights): the inner preconditioner becomes the line
                        factor followed by the judged network correction
@@ -35,7 +40,8 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Restart'``          Krylov basis size (30);
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
-                       preconditioner is refactored (5).
+                       preconditioner is refactored (5); a refinement step that
+                       stagnates on a lagged factor also refactors.
 
 Every solve appends (iterations, relative residual, preconditioner name,
 factorised?) to ``Solver.log`` for measurement.
@@ -72,6 +78,7 @@ def _opts(parameters):
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
         'Learned Network': o.get('Learned Network'),
         'Apply Graph': bool(o.get('Apply Graph', True)),
+        'Apply Storage': o.get('Apply Storage', 'fp16'),
     }
 
 
@@ -144,6 +151,12 @@ class LineFactor:
     def apply32(self, r, out):
         _lib.call(self._p + '_solve_f32', self._h, ptr(r), ptr(out), stream())
         return out
+
+    def set_half(self, on):
+        """FP32 apply from FP16 copies of the blocks (block cyclic reduction, FP32 factor only)."""
+        if self._p == 'tf_bcr' and self.single and bool(on) != getattr(self, 'half', False):
+            _lib.call('tf_bcr_set_half', self._h, 1 if on else 0, stream())
+            self.half = bool(on)
 
     def native(self):
         """(kind, handle, ncell, dof) for tf_fgmres."""
@@ -376,8 +389,14 @@ class Solver:
 
     def _build(self, jac, kind, single):
         f = self._build_prec(jac, kind, single)
+        opts = _opts(self.itf.parameters)
         if hasattr(f, 'set_graph'):
-            f.set_graph(_opts(self.itf.parameters)['Apply Graph'])
+            f.set_graph(opts['Apply Graph'])
+        if single and hasattr(f, 'set_half'):
+            storage = opts['Apply Storage']
+            if storage not in ('fp32', 'fp16'):
+                raise ValueError('unknown apply storage %r' % storage)
+            f.set_half(storage == 'fp16')
         return f
 
     def _build_prec(self, jac, kind, single):
@@ -535,7 +554,13 @@ class Solver:
                 rel = _norm(r, work) / bnrm
                 if rel <= opts['Tolerance']:
                     break
-                if rel > 0.5 * prev and jac.prec_exact:
+                if rel > 0.5 * prev and not jac.prec_exact:
+                    # the lagged factor no longer contracts: factor this Jacobian
+                    jac.prec = None
+                    self._lagged = None
+                    prec, fresh = self.preconditioner(jac, opts)
+                    prev = math.inf
+                elif rel > 0.5 * prev and jac.prec_exact:
                     # refinement stagnates with this Jacobian's own FP32 factor (FP32 too
                     # coarse for its conditioning): finish in FP64 from the current x
                     its64, rel, prec, _ = self._fp64_fallback(jac, b, x, opts)
