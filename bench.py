@@ -74,12 +74,15 @@ VARIANTS = {
     'fp64': {'Precision': 'fp64'},
     'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
     'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
+    'mixed16bg': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True},
 }
 DTYPES = {
     'fp64': 'f64',
     'mixed': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
     'mixed16': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies '
                'of the factor blocks with f32 row scales)',
+    'mixed16bg': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor factored on a side stream; '
+                 'apply reads f16 copies of the factor blocks with f32 row scales)',
 }
 
 
@@ -106,10 +109,12 @@ def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     fact = sum(1 for entry in itf.solver.log if entry[3])
     gm = sum(entry[0] for entry in itf.solver.log)
     worst = max(entry[1] for entry in itf.solver.log)
+    bg = itf.solver._bg
     return {
         'itf': itf, 'x': x, 'mu': mu, 'seconds': dt, 'accepted': acc, 'corrector_iterations': iters,
         'initial_newton_iterations': newton0, 'solves': len(itf.solver.log), 'factorizations': fact,
         'gmres_iterations': gm, 'worst_relative_residual': worst,
+        'background_factorizations': bg.started if bg is not None else 0,
         'steps_per_s': acc / dt,
     }
 
@@ -215,8 +220,10 @@ def cpu_baseline(n, x_host, mu, iters_total, accepted, reps=3):
 def kernel_sample(n, reps=10):
     """Short launch sequence for hardware-counter runs: one FP32 line
     factorisation of the n x n cavity Jacobian at Re = 1000, ``reps`` FP32
-    applies, and a plain copy of known size (out = x - y over 64 Mi floats:
-    512 MiB read, 256 MiB written) to calibrate FETCH_SIZE / WRITE_SIZE."""
+    applies reading the FP16 copies of the factor blocks (the default inner
+    apply of the mixed-precision solver), and a plain copy of known size
+    (out = x - y over 64 Mi floats: 512 MiB read, 256 MiB written) to calibrate
+    FETCH_SIZE / WRITE_SIZE."""
     import torch
     import transiflow_amd as tf
     from transiflow_amd import _lib
@@ -350,12 +357,13 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        key = '%s %s' % (name, {'fp64': 'f64', 'mixed': 'f32', 'mixed16': 'f16'}[head])
+        key = '%s %s' % (name, {'fp64': 'f64', 'mixed': 'f32', 'mixed16': 'f16', 'mixed16bg': 'f16'}[head])
         if key in tj:
             traffic = tj[key].get('bytes_per_launch')
     roof = {'bound': bound, 'achieved': achieved, 'peak': peak, 'unit': unit, 'frac': achieved / peak,
             'traffic': traffic,
-            'kernel': '%s (%s)' % (name, {'fp64': 'FP64', 'mixed': 'FP32', 'mixed16': 'FP32, FP16 blocks'}[head]),
+            'kernel': '%s (%s)' % (name, {'fp64': 'FP64', 'mixed': 'FP32', 'mixed16': 'FP32, FP16 blocks',
+                                          'mixed16bg': 'FP32, FP16 blocks'}[head]),
             'launches': calls, 'avg_launch_us': ms / max(calls, 1) * 1e3,
             'per_launch_work': work / max(calls, 1), 'work_unit': wunit,
             'profiled_run_steps_per_s': pr['steps_per_s'],
@@ -388,7 +396,8 @@ def main():
         },
         'variants': {v: {k: r[k] for k in ('steps_per_s', 'seconds', 'accepted', 'corrector_iterations',
                                            'factorizations', 'solves', 'gmres_iterations',
-                                           'worst_relative_residual')} for v, r in results.items()},
+                                           'worst_relative_residual', 'background_factorizations')}
+                     for v, r in results.items()},
         'final_state_rel_err_vs_fp64': agree,
         'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},
         'assembly': asm,

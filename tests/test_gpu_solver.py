@@ -153,6 +153,32 @@ def test_half_apply(hip, shape):
     P.set_graph(False)
 
 
+def test_background_factor_continuation(hip):
+    """Mixed precision with the next factor made on a side stream while the inner
+    solves run with the lagged one: the continuation reaches the FP64 method's
+    final state to 1e-8, every solve meets the 1e-10 bar, and background factors
+    were taken into use."""
+    import numpy as np
+    out = {}
+    for label, p in (('fp64', {'Precision': 'fp64'}),
+                     ('bg', {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True})):
+        itf = hip.Interface({'HIP Solver': p}, 64, 64)
+        c = hip.Continuation(itf, newton_tolerance=1e-8, printer=None)
+        x0 = c.newton(itf.vector(), tol=1e-8)
+        x, mu = c.continuation(x0, 'Reynolds Number', 0, 1000, 100)
+        out[label] = (itf.array_from_vector(x), mu, itf)
+    xb, mub, itf = out['bg']
+    xf, muf, _ = out['fp64']
+    err = np.linalg.norm(xb - xf) / np.linalg.norm(xf)
+    bg = itf.solver._bg
+    print('background factor: %d started, %d taken into use, %d solves, final state rel err %.2e'
+          % (bg.started, bg.swaps, len(itf.solver.log), err), flush=True)
+    assert abs(mub - muf) <= 1e-8 * max(1.0, abs(muf))
+    assert err <= 1e-8
+    assert bg.swaps >= 1
+    assert max(entry[1] for entry in itf.solver.log) <= 1e-10
+
+
 def test_solve_mixed_fallback(hip):
     """At a state of unit magnitude the line blocks reach condition numbers ~1e7 and
     the FP32 factor cannot refine: the solve finishes in FP64 and still meets the

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_sp_cols(int m, const Cpl* __restrict__ 
 //
 // Every solve kernel is a batch of row-times-vector products over m-long rows.
 // A group of lanes owns one row and reads it with 16-byte loads when the row
-// length and the vectors allow it (see RowGroup for the group size).  The rows
+// length and the vectors allow it (group_for sets the group size).  The rows
 // come from the factor's own blocks (Blocks<T>), or from FP16 copies of the
 // FP32 blocks with one power-of-two FP32 scale per row (HalfBlocks): half the
 // bytes for the FP32 apply of the mixed-precision solver.
@@ -389,18 +389,15 @@ struct Vec16<double> {
   static constexpr int W = 2;
 };
 
-// Lanes per row: 16 for FP32/FP64 rows (measured: 8-lane groups for FP32 rows
-// are slower); the FP16 copies use GroupOf<HalfBlocks>.
-template <int G_>
-struct RowGroup {
-  static constexpr int G = G_;
-  static constexpr int ROWS = 256 / G;
-};
-constexpr int HALF_G = 16;
+// Lanes per row G (16, 32 or 64; a launch argument): 16 when a level has many
+// lines (measured: 8-lane groups are slower); wider groups for the few lines of
+// the upper levels, where each group's chain of dependent loads sets the time.
+__host__ __device__ __forceinline__ int rows_per_block(int G) { return 256 / G; }
+inline int group_for(int batch) { return batch >= 32 ? 16 : (batch >= 8 ? 32 : 64); }
 
 // sum over c of a[c] x[c] by the G-lane group g (the result is in every lane of the group)
-template <int G, typename T, typename V>
-__device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g,
+template <typename T, typename V>
+__device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g, int G,
                                             bool vec) {
   double acc = 0.0;
   if constexpr (sizeof(T) == sizeof(V)) {
@@ -426,9 +423,8 @@ __device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __
 }
 
 // FP16 row against an FP32 vector: 8 halves (16 bytes) and two float4 per lane per step
-template <int G>
 __device__ __forceinline__ double group_dot(const f16* __restrict__ a, const float* __restrict__ x, int m, int g,
-                                            bool vec) {
+                                            int G, bool vec) {
   double acc = 0.0;
   if (vec) {
     const f16x8* av = (const f16x8*)a;
@@ -467,80 +463,82 @@ struct HalfBlocks {
   __device__ __forceinline__ double scale(int b, int r) const { return (double)s[b][r]; }
 };
 
-template <typename B>
-struct GroupOf : RowGroup<16> {};
-template <>
-struct GroupOf<HalfBlocks> : RowGroup<HALF_G> {};
-
 template <typename B, typename V>
-__device__ __forceinline__ double row_dot(const B& blk, int b, int r, const V* __restrict__ x, int m, int g,
+__device__ __forceinline__ double row_dot(const B& blk, int b, int r, const V* __restrict__ x, int m, int g, int G,
                                           bool vec) {
-  return blk.scale(b, r) * group_dot<GroupOf<B>::G>(blk.row(b, r, m), x, m, g, vec);
+  return blk.scale(b, r) * group_dot(blk.row(b, r, m), x, m, g, G, vec);
 }
 
 // rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines
 template <typename B, typename V>
-__global__ __launch_bounds__(256) void k_fwd(int m, int vec, const int* __restrict__ k_line,
+__global__ __launch_bounds__(256) void k_fwd(int m, int G, int vec, const int* __restrict__ k_line,
                                              const int* __restrict__ e1, const int* __restrict__ e2, B P1, B P2,
                                              V* __restrict__ rw) {
-  constexpr int G = GroupOf<B>::G;
   const int b = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   double acc = 0.0;
-  if (e1[b] >= 0) acc += row_dot(P1, b, rr, rw + (long)e1[b] * m, m, g, vec);
-  if (e2[b] >= 0) acc += row_dot(P2, b, rr, rw + (long)e2[b] * m, m, g, vec);
+  if (e1[b] >= 0) acc += row_dot(P1, b, rr, rw + (long)e1[b] * m, m, g, G, vec);
+  if (e2[b] >= 0) acc += row_dot(P2, b, rr, rw + (long)e2[b] * m, m, g, G, vec);
   if (live && g == 0) {
     V* out = rw + (long)k_line[b] * m + r;
     *out = (V)((double)*out - acc);
   }
 }
 
-// t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, or the level-0 sparse rows)
+// t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, levels >= 1)
 template <typename B, typename V>
-__global__ __launch_bounds__(256) void k_bwd_t(int m, int vec, int sparse, const int* __restrict__ e_line,
+__global__ __launch_bounds__(256) void k_bwd_t(int m, int G, int vec, const int* __restrict__ e_line,
                                                const int* __restrict__ a_line, const int* __restrict__ b_line,
-                                               B Lb, B Ub, const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
-                                               const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
-                                               const V* __restrict__ rw, const V* __restrict__ x,
+                                               B Lb, B Ub, const V* __restrict__ rw, const V* __restrict__ x,
                                                V* __restrict__ tbuf) {
-  constexpr int G = GroupOf<B>::G;
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long row = (long)e_line[bi] * m + rr;
   const int a = a_line[bi], bb = b_line[bi];
   double acc = 0.0;
-  if (sparse) {
-    if (live && g == 0) {
-      if (a >= 0)
-        for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)x[(long)a * m + Lr[row * KC + q].idx];
-      if (bb >= 0)
-        for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
-    }
-  } else {
-    if (a >= 0) acc += row_dot(Lb, bi, rr, x + (long)a * m, m, g, vec);
-    if (bb >= 0) acc += row_dot(Ub, bi, rr, x + (long)bb * m, m, g, vec);
-  }
+  if (a >= 0) acc += row_dot(Lb, bi, rr, x + (long)a * m, m, g, G, vec);
+  if (bb >= 0) acc += row_dot(Ub, bi, rr, x + (long)bb * m, m, g, G, vec);
   if (live && g == 0) tbuf[row] = (V)((double)rw[row] - acc);
+}
+
+// Level 0: t[e] = rw[e] - L_e x[a] - U_e x[b] with the sparse couplings.  Thread per row.
+template <typename V>
+__global__ __launch_bounds__(256) void k_bwd_t_sparse(int m, const int* __restrict__ e_line,
+                                                      const int* __restrict__ a_line, const int* __restrict__ b_line,
+                                                      const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
+                                                      const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
+                                                      const V* __restrict__ rw, const V* __restrict__ x,
+                                                      V* __restrict__ tbuf) {
+  const int bi = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const long row = (long)e_line[bi] * m + r;
+  const int a = a_line[bi], bb = b_line[bi];
+  double acc = 0.0;
+  if (a >= 0)
+    for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)x[(long)a * m + Lr[row * KC + q].idx];
+  if (bb >= 0)
+    for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
+  tbuf[row] = (V)((double)rw[row] - acc);
 }
 
 // x[e] = Y_e t[e]
 template <typename B, typename V>
-__global__ __launch_bounds__(256) void k_bwd_x(int m, int vec, const int* __restrict__ e_line, B Yb,
+__global__ __launch_bounds__(256) void k_bwd_x(int m, int G, int vec, const int* __restrict__ e_line, B Yb,
                                                const V* __restrict__ tbuf, V* __restrict__ x) {
-  constexpr int G = GroupOf<B>::G;
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * GroupOf<B>::ROWS + threadIdx.x / G;
+  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
-  const double acc = row_dot(Yb, bi, rr, tbuf + base, m, g, vec);
+  const double acc = row_dot(Yb, bi, rr, tbuf + base, m, g, G, vec);
   if (live && g == 0) x[base + r] = (V)acc;
 }
 
@@ -836,7 +834,8 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
            B Ub(const BcrLevelDev&), B Yb(const BcrLevelDev&), V* rw, V* tb, V* xo, hipStream_t st) {
   const int m = h->m;
   const dim3 blk(256);
-  const unsigned gx = (unsigned)((m + GroupOf<B>::ROWS - 1) / GroupOf<B>::ROWS);
+  auto gx = [m](int G) { return (unsigned)((m + rows_per_block(G) - 1) / rows_per_block(G)); };
+  const unsigned gs = (unsigned)((m + 255) / 256);
   V* yb = (V*)h->yb;
   for (size_t l = 0; l < h->lev.size(); ++l) {
     const BcrLevelDev& L = h->lev[l];
@@ -844,19 +843,25 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
     if (l == 0) {
       // the level-0 couplings are sparse: y_e = Y_e r_e, then r_k -= L_k y_e1 + U_k y_e2
       // (one Y block per eliminated line instead of two multipliers per kept line)
-      hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, Yb(L), (const V*)rw, yb);
-      hipLaunchKernelGGL((k_fwd_sparse<V>), dim3((m + 255) / 256, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr,
-                         h->nLr, h->Ur, h->nUr, (const V*)yb, rw);
+      const int G = group_for(L.ne);
+      hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)rw, yb);
+      hipLaunchKernelGGL((k_fwd_sparse<V>), dim3(gs, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr, h->nLr, h->Ur,
+                         h->nUr, (const V*)yb, rw);
     } else {
-      hipLaunchKernelGGL((k_fwd<B, V>), dim3(gx, L.nk), blk, 0, st, m, vec, L.k, L.k1, L.k2, P1(L), P2(L), rw);
+      const int G = group_for(L.nk);
+      hipLaunchKernelGGL((k_fwd<B, V>), dim3(gx(G), L.nk), blk, 0, st, m, G, vec, L.k, L.k1, L.k2, P1(L), P2(L), rw);
     }
   }
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
-    // level 0 uses the sparse couplings (no dense L/U rows are read)
-    hipLaunchKernelGGL((k_bwd_t<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, l == 0 ? 1 : 0, L.e, L.ea, L.eb, Lb(L),
-                       Ub(L), h->Lr, h->nLr, h->Ur, h->nUr, (const V*)rw, (const V*)xo, tb);
-    hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx, L.ne), blk, 0, st, m, vec, L.e, Yb(L), (const V*)tb, xo);
+    const int G = group_for(L.ne);
+    if (l == 0)
+      hipLaunchKernelGGL((k_bwd_t_sparse<V>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr, h->Ur,
+                         h->nUr, (const V*)rw, (const V*)xo, tb);
+    else
+      hipLaunchKernelGGL((k_bwd_t<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Lb(L), Ub(L),
+                         (const V*)rw, (const V*)xo, tb);
+    hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)tb, xo);
   }
 }
 

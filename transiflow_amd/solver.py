@@ -41,7 +41,15 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
                        preconditioner is refactored (5); a refinement step that
-                       stagnates on a lagged factor also refactors.
+                       stagnates on a lagged factor also refactors;
+``'Background Factor'`` (mixed, line factor) False (default) or True: when the
+                       lagged factor exceeds the reuse limit, a second handle
+                       factors the Jacobian on a side stream while the inner
+                       solves go on with the lagged factor, and replaces it
+                       between refinement steps once finished (the compute-bound
+                       factorisation overlaps the HBM-bound applies; iteration
+                       counts then depend on timing, results still meet
+                       ``'Tolerance'``).
 
 Every solve appends (iterations, relative residual, preconditioner name,
 factorised?) to ``Solver.log`` for measurement.
@@ -79,6 +87,7 @@ def _opts(parameters):
         'Learned Network': o.get('Learned Network'),
         'Apply Graph': bool(o.get('Apply Graph', True)),
         'Apply Storage': o.get('Apply Storage', 'fp16'),
+        'Background Factor': bool(o.get('Background Factor', False)),
     }
 
 
@@ -113,7 +122,7 @@ class LineFactor:
     Thomas recursion (``'thomas'``, csrc/linelu.hip)."""
     _ABI = {'line': 'tf_bcr', 'thomas': 'tf_linelu'}
 
-    def __init__(self, itf, jac, gauge, single=False, kind='line'):
+    def __init__(self, itf, jac, gauge, single=False, kind='line', factor=True):
         self.name = kind
         self._p = self._ABI[kind]
         h = ctypes.c_void_p()
@@ -127,18 +136,27 @@ class LineFactor:
         self.single = single
         self.m = m
         self.owner = None
-        self.refactor(jac, gauge)
+        if factor:
+            self.refactor(jac, gauge)
 
     def refactor(self, jac, gauge):
         """Factor ``jac`` into this handle's buffers (the previous owner's factor is gone)."""
+        self.refactor_async(jac, gauge, stream())
+        self.check(stream())
+
+    def refactor_async(self, jac, gauge, st):
+        """Queue the factorisation of ``jac`` on stream ``st`` (a ctypes stream pointer)."""
         old = self.owner() if self.owner is not None else None
         if old is not None and old is not jac and old.prec is self:
             old.prec = None
             old.prec_exact = False
         self.owner = weakref.ref(jac)
-        _lib.call(self._p + '_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), stream())
+        _lib.call(self._p + '_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), st)
+
+    def check(self, st):
+        """Structure flags of the last factorisation (synchronises ``st``)."""
         flags = (ctypes.c_int * 2)()
-        _lib.call(self._p + '_status', self._h, flags, stream())
+        _lib.call(self._p + '_status', self._h, flags, st)
         if flags[0]:
             raise SolverError('line factorisation: matrix is not block tridiagonal over grid lines (flags %d)'
                               % flags[0])
@@ -362,6 +380,107 @@ def _norm(x, work):
     return out.value
 
 
+class BackgroundFactor:
+    """Two FP32 line-factor handles for the mixed-precision solve: the inner
+    solves use ``cur`` (the factor of an earlier Jacobian, or of this one)
+    while the other handle factors the newest Jacobian on a side stream; a
+    finished factor becomes ``cur`` between refinement steps.  Before it starts,
+    the side stream waits for everything the solve stream has queued (the
+    Jacobian's assembly and the last applies of the handle it overwrites), and
+    the Jacobian is kept alive until the factorisation has finished."""
+
+    def __init__(self, solver, kind):
+        self.solver = solver
+        self.kind = kind
+        self.cur = None
+        self.spare = None
+        self.pending = None     # (factor, jac, done event)
+        self.users = []         # weak references of Jacobians whose prec is cur
+        self.stream = torch().cuda.Stream()
+        self.started = 0        # background factorisations queued
+        self.swaps = 0          # ... and taken into use
+
+    def _configure(self, f):
+        opts = _opts(self.solver.itf.parameters)
+        f.set_graph(opts['Apply Graph'])
+        f.set_half(opts['Apply Storage'] == 'fp16')
+        return f
+
+    def _adopt(self, f, jac, exact):
+        """Make ``f`` the current factor; Jacobians that held the old one drop it."""
+        old = self.cur
+        self.cur, self.spare = f, old
+        for ref in self.users:
+            j = ref()
+            if j is not None and j.prec is old:
+                j.prec = None
+                j.prec_exact = False
+        self.users = []
+        if jac is not None:
+            self.assign(jac, exact)
+
+    def assign(self, jac, exact=None):
+        jac.prec = self.cur
+        jac.prec_exact = (self.cur.owner is not None and self.cur.owner() is jac) if exact is None else exact
+        self.users.append(weakref.ref(jac))
+
+    def poll(self, jac=None, wait=False):
+        """Take a finished background factor into use; True if one was."""
+        p = self.pending
+        if p is None:
+            return False
+        f, fjac, done = p
+        if not wait and not done.query():
+            return False
+        done.synchronize()
+        f.check(ctypes.c_void_p(self.stream.cuda_stream))
+        self.pending = None
+        self._adopt(f, None, False)
+        for j in (fjac, jac):
+            if j is not None and j.prec is None:
+                self.assign(j)
+        self.swaps += 1
+        return True
+
+    def start(self, jac):
+        """Factor ``jac`` on the side stream unless a factorisation is running or
+        the current factor is already this Jacobian's."""
+        if self.pending is not None or self.cur is None:
+            return
+        if self.cur.owner is not None and self.cur.owner() is jac:
+            return
+        s = self.solver
+        if self.spare is None:
+            self.spare = self._configure(LineFactor(s.itf, jac, s.gauge(), single=True, kind=self.kind,
+                                                    factor=False))
+        T = torch()
+        queued = T.cuda.Event()
+        queued.record()
+        self.stream.wait_event(queued)
+        self.spare.refactor_async(jac, s.gauge(), ctypes.c_void_p(self.stream.cuda_stream))
+        done = T.cuda.Event()
+        done.record(self.stream)
+        self.pending = (self.spare, jac, done)
+        self.spare = None
+        self.started += 1
+
+    def now(self, jac):
+        """The factor of ``jac`` itself, made on the solve stream if no background
+        factorisation provides it."""
+        self.poll(jac, wait=True)
+        if self.cur is not None and self.cur.owner is not None and self.cur.owner() is jac:
+            self.assign(jac, True)
+            return self.cur
+        s = self.solver
+        if self.spare is None:
+            f = self._configure(LineFactor(s.itf, jac, s.gauge(), single=True, kind=self.kind))
+        else:
+            f = self.spare
+            f.refactor(jac, s.gauge())
+        self._adopt(f, jac, True)
+        return f
+
+
 class Solver:
     def __init__(self, itf):
         self.itf = itf
@@ -375,6 +494,7 @@ class Solver:
         self._lagged = None   # (precond) reused across Jacobians in mixed precision
         self._buf = None
         self._factors = {}    # (kind, single) -> LineFactor, refactored in place
+        self._bg = None       # BackgroundFactor (mixed, 'Background Factor')
 
     def _krylov_for(self, restart, dtype='float64', native=False):
         key = (dtype, native)
@@ -417,8 +537,29 @@ class Solver:
             return CellBlockJacobi(self.itf, jac, g)
         raise ValueError('unknown preconditioner %r' % kind)
 
+    def _background(self, opts):
+        """The BackgroundFactor when enabled and the preconditioner is a line factor."""
+        if opts['Precision'] != 'mixed' or not opts['Background Factor']:
+            return None
+        if self._bg is None:
+            kind = opts['Preconditioner']
+            if kind == 'auto':
+                kind = 'line' if line_fits(self.itf, True) else 'block-jacobi'
+            if kind not in ('line', 'thomas'):
+                return None
+            self._bg = BackgroundFactor(self, kind)
+        return self._bg
+
     def preconditioner(self, jac, opts):
         mixed = opts['Precision'] == 'mixed'
+        bg = self._background(opts)
+        if bg is not None:
+            bg.poll(jac)
+            if bg.cur is None:
+                return bg.now(jac), True
+            if jac.prec is None:
+                bg.assign(jac)
+            return jac.prec, False
         if jac.prec is not None:
             return jac.prec, False
         if mixed and self._lagged is not None:
@@ -502,6 +643,15 @@ class Solver:
             its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
         return its, rel, prec, True
 
+    def _refactor_now(self, jac, opts):
+        """Replace a lagged factor by this Jacobian's own (mixed precision)."""
+        bg = self._background(opts)
+        if bg is not None:
+            return bg.now(jac), True
+        jac.prec = None
+        self._lagged = None
+        return self.preconditioner(jac, opts)
+
     def _buffers(self, n):
         T = torch()
         if self._buf is None or self._buf[0].numel() != n:
@@ -519,6 +669,7 @@ class Solver:
         kry = self._krylov_for(opts['Restart'], 'float32')
         work = self._krylov_for(opts['Restart']).work
         val32 = jac.values32()
+        bg = self._background(opts)
         prec, fresh = self.preconditioner(jac, opts)
 
         def matvec32(v, out):
@@ -554,11 +705,11 @@ class Solver:
                 rel = _norm(r, work) / bnrm
                 if rel <= opts['Tolerance']:
                     break
+                if bg is not None and bg.poll(jac):
+                    prec = jac.prec
                 if rel > 0.5 * prev and not jac.prec_exact:
                     # the lagged factor no longer contracts: factor this Jacobian
-                    jac.prec = None
-                    self._lagged = None
-                    prec, fresh = self.preconditioner(jac, opts)
+                    prec, fresh = self._refactor_now(jac, opts)
                     prev = math.inf
                 elif rel > 0.5 * prev and jac.prec_exact:
                     # refinement stagnates with this Jacobian's own FP32 factor (FP32 too
@@ -587,10 +738,12 @@ class Solver:
             outer += 1
             if not jac.prec_exact and its > opts['Reuse Limit']:
                 # lagged factor too stale: refactor this Jacobian and keep refining
-                jac.prec = None
-                self._lagged = None
-                prec, fresh = self.preconditioner(jac, opts)
-                prev = math.inf
+                # (in the background: with the lagged factor until the new one is ready)
+                if bg is not None:
+                    bg.start(jac)
+                else:
+                    prec, fresh = self._refactor_now(jac, opts)
+                    prev = math.inf
         if outer >= opts['Maximum Refinements'] or total >= opts['Maximum Iterations']:
             jac.matvec(x, out=r, gauge=g)
             _lib.call('tf_axpby_f64', n, 1.0, ptr(b), -1.0, ptr(r), ptr(r), st)
