@@ -75,6 +75,8 @@ VARIANTS = {
     'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
     'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
     'mixed16bg': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True},
+    'mixed16s3': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16x3'},
+    'mixed16s1': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16'},
 }
 DTYPES = {
     'fp64': 'f64',
@@ -83,6 +85,10 @@ DTYPES = {
                'of the factor blocks with f32 row scales)',
     'mixed16bg': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor factored on a side stream; '
                  'apply reads f16 copies of the factor blocks with f32 row scales)',
+    'mixed16s3': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor with its level GEMMs on '
+                 'bf16 MFMA over three-term bf16 splits; apply reads f16 copies with f32 row scales)',
+    'mixed16s1': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor with its level GEMMs on '
+                 'bf16 MFMA over bf16-rounded operands; apply reads f16 copies with f32 row scales)',
 }
 
 
@@ -357,13 +363,13 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        key = '%s %s' % (name, {'fp64': 'f64', 'mixed': 'f32', 'mixed16': 'f16', 'mixed16bg': 'f16'}[head])
+        key = '%s %s' % (name, 'f64' if head == 'fp64' else ('f32' if head == 'mixed' else 'f16'))
         if key in tj:
             traffic = tj[key].get('bytes_per_launch')
     roof = {'bound': bound, 'achieved': achieved, 'peak': peak, 'unit': unit, 'frac': achieved / peak,
             'traffic': traffic,
-            'kernel': '%s (%s)' % (name, {'fp64': 'FP64', 'mixed': 'FP32', 'mixed16': 'FP32, FP16 blocks',
-                                          'mixed16bg': 'FP32, FP16 blocks'}[head]),
+            'kernel': '%s (%s)' % (name, 'FP64' if head == 'fp64' else ('FP32' if head == 'mixed'
+                                                                        else 'FP32, FP16 blocks')),
             'launches': calls, 'avg_launch_us': ms / max(calls, 1) * 1e3,
             'per_launch_work': work / max(calls, 1), 'work_unit': wunit,
             'profiled_run_steps_per_s': pr['steps_per_s'],

@@ -176,6 +176,10 @@ int tf_bcr_set_graph(tf_bcr* h, int on);
  * (no reference counterpart; the reference's splu factor is FP64).  The first call
  * allocates the copies; an existing factor is converted on ``stream``. */
 int tf_bcr_set_half(tf_bcr* h, int on, void* stream);
+/* split 0 (default): the level GEMMs of a single-precision factor run on FP32 MFMA; 3: on BF16
+ * MFMA over three-term BF16 splits of the FP32 operands (hi*hi + hi*lo + lo*hi, FP32
+ * accumulation, about 2^-16 relative per product); 1: one term (no reference counterpart). */
+int tf_bcr_set_gemm_split(tf_bcr* h, int split);
 
 /* ---- learned preconditioner: convolutions on the cell grid (no reference counterpart) ----
  * Activations cell-major [ncell][C] in BF16 (uint16 storage); layer weights in the

@@ -19,10 +19,12 @@ GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
 pytestmark = pytest.mark.gpu
 
 
-def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64', amplitude=1.0, storage='fp32'):
+def _solve_case(hip, params, shape, dim, dof, seed, prec='line', precision='fp64', amplitude=1.0, storage='fp32',
+                factor_gemm='fp32'):
     from oracle.interface import Interface as OracleInterface
     p = dict(params)
-    p['HIP Solver'] = {'Preconditioner': prec, 'Precision': precision, 'Apply Storage': storage}
+    p['HIP Solver'] = {'Preconditioner': prec, 'Precision': precision, 'Apply Storage': storage,
+                       'Factor GEMM': factor_gemm}
     itf = hip.Interface(p, *shape, dim, dof)
     ref = OracleInterface(dict(params), *shape, dim, dof)
     state = seeded_state(itf.n, seed) * amplitude
@@ -93,6 +95,17 @@ def test_solve_mixed_fp16_copies(hip, shape, re):
     factor blocks: the FP64 bar is unchanged (1e-10 residual, splu to 1e-8)."""
     itf = _solve_case(hip, {'Reynolds Number': re}, shape, 2, 3, 12, precision='mixed', amplitude=0.1,
                       storage='fp16')
+    assert itf.solver.fallbacks == 0
+
+
+@pytest.mark.parametrize('factor_gemm', ['bf16x3', 'bf16'])
+@pytest.mark.parametrize('shape,re', [((64, 40, 1), 1000), ((256, 256, 1), 2000)])
+def test_solve_mixed_bf16_factor_gemm(hip, shape, re, factor_gemm):
+    """The FP32 factor with its level GEMMs on BF16 MFMA (three-term splits or plain
+    BF16 rounding of the operands) inside the same refinement: the FP64 bar is
+    unchanged (1e-10 residual, splu to 1e-8)."""
+    itf = _solve_case(hip, {'Reynolds Number': re}, shape, 2, 3, 12, precision='mixed', amplitude=0.1,
+                      storage='fp16', factor_gemm=factor_gemm)
     assert itf.solver.fallbacks == 0
 
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     'tf_bcr_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
     'tf_bcr_set_graph': (_c_int, [_vp, _c_int]),
     'tf_bcr_set_half': (_c_int, [_vp, _c_int, _vp]),
+    'tf_bcr_set_gemm_split': (_c_int, [_vp, _c_int]),
     'tf_fill_f32': (_c_int, [_c_long, _c_double, _vp, _vp]),
     'tf_axpby_f32': (_c_int, [_c_long, _c_double, _vp, _c_double, _vp, _vp, _vp]),
     'tf_scale_f32': (_c_int, [_c_long, _c_double, _vp, _vp, _vp]),

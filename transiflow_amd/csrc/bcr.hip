@@ -712,6 +712,7 @@ struct tf_bcr {
   int ncv;
   size_t slots;
   size_t solve_bytes_half;
+  int split;           // level GEMMs of an FP32 factor on BF16 terms: 0 off, 1 or 3 terms
 };
 
 namespace {
@@ -724,6 +725,23 @@ void gemm(int M, int N, int K, void** A, int lda, void** B, int ldb, void** C, i
   TF_PROF_BEGIN(tf::PROF_GEMM, st);
   tf::gemm_batched<T>(M, N, K, (T* const*)A, lda, (T* const*)B, ldb, (T* const*)C, ldc, alpha, beta, batch, st);
   TF_PROF_END(tf::PROF_GEMM, st);
+}
+
+// The level GEMMs of an FP32 factor, optionally on BF16 terms (tf_bcr_set_gemm_split).
+template <typename T>
+void level_gemm(tf_bcr* h, int M, int N, int K, void** A, int lda, void** B, int ldb, void** C, int ldc, double alpha,
+                double beta, int batch, hipStream_t st) {
+  if constexpr (sizeof(T) == 4) {
+    if (h->split && batch > 0) {
+      TF_PROF_WORK(tf::PROF_GEMM, 2.0 * M * N * K * batch);
+      TF_PROF_BEGIN(tf::PROF_GEMM, st);
+      tf::gemm_batched_split(h->split, M, N, K, (float* const*)A, lda, (float* const*)B, ldb, (float* const*)C, ldc,
+                             alpha, beta, batch, st);
+      TF_PROF_END(tf::PROF_GEMM, st);
+      return;
+    }
+  }
+  gemm<T>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, batch, st);
 }
 
 template <typename T>
@@ -804,12 +822,12 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
                            L.k_fb_line,
                            (T* const*)L.g_fb_A, (T* const*)L.g_fb_C, -1.0, 0);
     } else {
-      gemm<T>(m, m, m, L.g_p1_A, m, L.g_p1_B, m, L.g_p1_C, m, 1.0, 0.0, L.n_p1, st);
-      gemm<T>(m, m, m, L.g_p2_A, m, L.g_p2_B, m, L.g_p2_C, m, 1.0, 0.0, L.n_p2, st);
-      gemm<T>(m, m, m, L.g_p1_C, m, L.g_d1_B, m, L.g_d1_A, m, -1.0, 1.0, L.n_p1, st);
-      gemm<T>(m, m, m, L.g_p2_C, m, L.g_d2_B, m, L.g_d2_A, m, -1.0, 1.0, L.n_p2, st);
-      gemm<T>(m, m, m, L.g_fa_A, m, L.g_fa_B, m, L.g_fa_C, m, -1.0, 0.0, L.n_fa, st);
-      gemm<T>(m, m, m, L.g_fb_A, m, L.g_fb_B, m, L.g_fb_C, m, -1.0, 0.0, L.n_fb, st);
+      level_gemm<T>(h, m, m, m, L.g_p1_A, m, L.g_p1_B, m, L.g_p1_C, m, 1.0, 0.0, L.n_p1, st);
+      level_gemm<T>(h, m, m, m, L.g_p2_A, m, L.g_p2_B, m, L.g_p2_C, m, 1.0, 0.0, L.n_p2, st);
+      level_gemm<T>(h, m, m, m, L.g_p1_C, m, L.g_d1_B, m, L.g_d1_A, m, -1.0, 1.0, L.n_p1, st);
+      level_gemm<T>(h, m, m, m, L.g_p2_C, m, L.g_d2_B, m, L.g_d2_A, m, -1.0, 1.0, L.n_p2, st);
+      level_gemm<T>(h, m, m, m, L.g_fa_A, m, L.g_fa_B, m, L.g_fa_C, m, -1.0, 0.0, L.n_fa, st);
+      level_gemm<T>(h, m, m, m, L.g_fb_A, m, L.g_fb_B, m, L.g_fb_C, m, -1.0, 0.0, L.n_fb, st);
     }
     // couplings that no longer exist
     const long mm = (long)m * m;
@@ -1367,6 +1385,16 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
                        (f16* const*)h->cv_dst, (float* const*)h->cv_scl);
     TF_LAUNCH_CHECK("bcr FP16 copies");
   }
+  return 0;
+}
+
+// split 0 (default): the level GEMMs of an FP32 factor use FP32 MFMA; 3: BF16 MFMA on
+// three-term splits of the FP32 operands (about 2^-16 relative per product); 1: one term.
+int tf_bcr_set_gemm_split(tf_bcr* h, int split) {
+  if (!h) return tf::fail("null handle");
+  if (split != 0 && split != 1 && split != 3) return tf::fail("split must be 0, 1 or 3");
+  if (split && !h->single) return tf::fail("BF16 GEMM terms need a single-precision factor");
+  h->split = split;
   return 0;
 }
 

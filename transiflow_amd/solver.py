@@ -42,6 +42,10 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
                        preconditioner is refactored (5); a refinement step that
                        stagnates on a lagged factor also refactors;
+``'Factor GEMM'``      (mixed, line factor) ``'fp32'`` (default), ``'bf16x3'`` or
+                       ``'bf16'``: the level GEMMs of the FP32 factor on FP32
+                       MFMA, or on BF16 MFMA over three-term (one-term) BF16
+                       splits of the FP32 operands;
 ``'Background Factor'`` (mixed, line factor) False (default) or True: when the
                        lagged factor exceeds the reuse limit, a second handle
                        factors the Jacobian on a side stream while the inner
@@ -88,7 +92,22 @@ def _opts(parameters):
         'Apply Graph': bool(o.get('Apply Graph', True)),
         'Apply Storage': o.get('Apply Storage', 'fp16'),
         'Background Factor': bool(o.get('Background Factor', False)),
+        'Factor GEMM': o.get('Factor GEMM', 'fp32'),
     }
+
+
+_SPLITS = {'fp32': 0, 'bf16x3': 3, 'bf16': 1}
+
+
+def _configure_single(f, opts):
+    """Apply-storage and GEMM options of an FP32 line factor."""
+    storage = opts['Apply Storage']
+    if storage not in ('fp32', 'fp16'):
+        raise ValueError('unknown apply storage %r' % storage)
+    if opts['Factor GEMM'] not in _SPLITS:
+        raise ValueError('unknown factor GEMM %r' % opts['Factor GEMM'])
+    f.set_half(storage == 'fp16')
+    f.set_gemm_split(_SPLITS[opts['Factor GEMM']])
 
 
 class CellBlockJacobi:
@@ -175,6 +194,12 @@ class LineFactor:
         if self._p == 'tf_bcr' and self.single and bool(on) != getattr(self, 'half', False):
             _lib.call('tf_bcr_set_half', self._h, 1 if on else 0, stream())
             self.half = bool(on)
+
+    def set_gemm_split(self, split):
+        """Level GEMMs of the FP32 factor on BF16 terms (0 off, 1 or 3 terms); applies
+        from the next factorisation on."""
+        if self._p == 'tf_bcr' and self.single:
+            _lib.call('tf_bcr_set_gemm_split', self._h, int(split))
 
     def native(self):
         """(kind, handle, ncell, dof) for tf_fgmres."""
@@ -403,7 +428,7 @@ class BackgroundFactor:
     def _configure(self, f):
         opts = _opts(self.solver.itf.parameters)
         f.set_graph(opts['Apply Graph'])
-        f.set_half(opts['Apply Storage'] == 'fp16')
+        _configure_single(f, opts)
         return f
 
     def _adopt(self, f, jac, exact):
@@ -472,11 +497,10 @@ class BackgroundFactor:
             self.assign(jac, True)
             return self.cur
         s = self.solver
-        if self.spare is None:
-            f = self._configure(LineFactor(s.itf, jac, s.gauge(), single=True, kind=self.kind))
-        else:
-            f = self.spare
-            f.refactor(jac, s.gauge())
+        f = self.spare
+        if f is None:
+            f = self._configure(LineFactor(s.itf, jac, s.gauge(), single=True, kind=self.kind, factor=False))
+        f.refactor(jac, s.gauge())
         self._adopt(f, jac, True)
         return f
 
@@ -513,10 +537,7 @@ class Solver:
         if hasattr(f, 'set_graph'):
             f.set_graph(opts['Apply Graph'])
         if single and hasattr(f, 'set_half'):
-            storage = opts['Apply Storage']
-            if storage not in ('fp32', 'fp16'):
-                raise ValueError('unknown apply storage %r' % storage)
-            f.set_half(storage == 'fp16')
+            _configure_single(f, opts)
         return f
 
     def _build_prec(self, jac, kind, single):
@@ -529,9 +550,11 @@ class Solver:
             # of a fresh allocation (the factor of the previous Jacobian goes)
             f = self._factors.get((kind, single))
             if f is None:
-                f = self._factors[(kind, single)] = LineFactor(self.itf, jac, g, single=single, kind=kind)
-            else:
-                f.refactor(jac, g)
+                f = self._factors[(kind, single)] = LineFactor(self.itf, jac, g, single=single, kind=kind,
+                                                               factor=False)
+                if single:
+                    _configure_single(f, _opts(self.itf.parameters))
+            f.refactor(jac, g)
             return f
         if kind == 'block-jacobi':
             return CellBlockJacobi(self.itf, jac, g)
