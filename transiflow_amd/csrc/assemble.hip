@@ -307,7 +307,7 @@ int tf_problem_create(tf_problem** out, int nx, int ny, int nz, int dim, int dof
   for (int a = 0; a < 3; ++a) {
     const size_t bytes = sizeof(double) * (p->n[a] + 3);
     TF_HIP(hipMalloc(&p->coord_d[a], bytes));
-    TF_HIP(hipMemcpy(p->coord_d[a], host[a], bytes, hipMemcpyHostToDevice));
+    TF_HIP(tf::upload(p->coord_d[a], host[a], bytes));
   }
   TF_HIP(hipMalloc(&p->ops_d, sizeof(BcOp) * MAX_BC_OPS));
   TF_HIP(hipMalloc(&p->counts_d, sizeof(int) * p->nrow));
@@ -355,7 +355,7 @@ int tf_problem_set_bcs(tf_problem* p, int nops, const int* kind, const int* axis
     if (kind[o] == BC_FORCE && (var[o] < 0 || var[o] >= p->dof)) return fail("bad forcing variable");
     ops[o] = BcOp{kind[o], axis[o], side[o], var[o], sign[o], fc[o], ac[o]};
   }
-  if (nops) TF_HIP(hipMemcpy(p->ops_d, ops, sizeof(BcOp) * nops, hipMemcpyHostToDevice));
+  if (nops) TF_HIP(tf::upload(p->ops_d, ops, sizeof(BcOp) * nops));
   p->ph.nops = nops;
   return 0;
 }

@@ -1178,7 +1178,7 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
   }
   pb.align();
   TF_HIP(hipMalloc(&h->pool, pb.bytes.size() + 64));
-  TF_HIP(hipMemcpy(h->pool, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice));
+  TF_HIP(tf::upload(h->pool, pb.bytes.data(), pb.bytes.size()));
   char* P = (char*)h->pool;
   for (const Off& o : offs) {
     BcrLevelDev L{};
@@ -1352,7 +1352,7 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
     const size_t ocs = pb.add_ptrs(cs), ocd = pb.add_ptrs(cd), occ = pb.add_ptrs(cc);
     pb.align();
     TF_HIP(hipMalloc(&h->hpool, pb.bytes.size() + 64));
-    TF_HIP(hipMemcpy(h->hpool, pb.bytes.data(), pb.bytes.size(), hipMemcpyHostToDevice));
+    TF_HIP(tf::upload(h->hpool, pb.bytes.data(), pb.bytes.size()));
     char* P = (char*)h->hpool;
     for (size_t l = 0; l < offs.size(); ++l) {
       BcrLevelDev& L = h->lev[l];

@@ -40,6 +40,28 @@ hipEvent_t take_event(ProfState& s) {
 
 bool prof_on() { return ps().on; }
 
+hipError_t upload(void* dst, const void* src, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<hipStream_t> streams;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipStream_t s = nullptr;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    if ((int)streams.size() <= dev) streams.resize(dev + 1, nullptr);
+    if (!streams[dev]) {
+      e = hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking);
+      if (e != hipSuccess) return e;
+    }
+    s = streams[dev];
+  }
+  if (bytes == 0) return hipSuccess;
+  e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
 void prof_work(int id, double w) {
   ProfState& s = ps();
   std::lock_guard<std::mutex> g(s.mu);

@@ -41,7 +41,7 @@ int time_candidates(int M, int N, int K, int batch, hipStream_t st) {
       h[2 * batch + i] = c;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipMemcpy(lists, h.data(), sizeof(T*) * h.size(), hipMemcpyHostToDevice) == hipSuccess &&
+    if (upload(lists, h.data(), sizeof(T*) * h.size()) == hipSuccess &&
         hipMemsetAsync(a, 0, sizeof(T) * (size_t)M * K, st) == hipSuccess &&
         hipMemsetAsync(b, 0, sizeof(T) * (size_t)K * N, st) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
         hipEventCreate(&e1) == hipSuccess) {

@@ -45,6 +45,10 @@ enum ProfId : int {
   PROF_IDS = 16
 };
 bool prof_on();
+// Host-to-device copy through a process-wide non-blocking stream (created once
+// per device), synchronised before returning.  Used instead of hipMemcpy, whose
+// legacy-stream semantics make it fail while another thread captures a graph.
+hipError_t upload(void* dst, const void* src, size_t bytes);
 void prof_begin(int id, hipStream_t st);
 void prof_end(int id, hipStream_t st);
 void prof_work(int id, double w);
