@@ -1002,6 +1002,9 @@ extern "C" {
 
 // ylines != 0 (2D only): lines are the grid columns i (m = ny dof rows each, nx
 // lines) instead of the rows j; n = nx ny dof.
+int tf_bcr_destroy(tf_bcr* h);
+static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof, int ylines);
+
 int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int ylines) {
   if (!out) return tf::fail("null handle");
   *out = nullptr;
@@ -1009,6 +1012,17 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
   if (ylines && (nx < 1 || dof < 1 || m % dof != 0 || (long)nx * m != n))
     return tf::fail("y-line order needs n = nx * m with m = ny * dof");
   tf_bcr* h = new tf_bcr();
+  const int rc = bcr_create_impl(h, n, m, single, nx, dof, ylines);
+  if (rc) {
+    // frees whatever was allocated before the failure (unset members are null)
+    tf_bcr_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof, int ylines) {
   h->order = Order{ylines ? 1 : 0, nx, dof, m};
   h->n = n;
   h->m = m;
@@ -1225,7 +1239,6 @@ int tf_bcr_create(tf_bcr** out, int n, int m, int single, int nx, int dof, int y
     L.z_fb = (void**)(P + o.zfb);
     h->lev.push_back(L);
   }
-  *out = h;
   return 0;
 }
 
@@ -1285,10 +1298,17 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
   const int m = h->m;
   const size_t mm = (size_t)m * m, N = h->N, slots = h->slots;
   if (on && !h->H) {
-    TF_HIP(hipMalloc(&h->H, sizeof(f16) * mm * (3 * N + 2 * slots)));
-    TF_HIP(hipMalloc(&h->S, sizeof(float) * (size_t)m * (3 * N + 2 * slots)));
-    f16* H = (f16*)h->H;
-    float* S = (float*)h->S;
+    // all three allocations succeed before any is kept (a failed call leaves no copies)
+    void *H_ = nullptr, *S_ = nullptr;
+    hipError_t e = hipMalloc(&H_, sizeof(f16) * mm * (3 * N + 2 * slots));
+    if (e == hipSuccess) e = hipMalloc(&S_, sizeof(float) * (size_t)m * (3 * N + 2 * slots));
+    if (e != hipSuccess) {
+      (void)hipFree(H_);
+      (void)hipFree(S_);
+      return tf::fail_hip(e, "hipMalloc (FP16 block copies)");
+    }
+    f16* H = (f16*)H_;
+    float* S = (float*)S_;
     // block j of buffer q (0 D, 1 L, 2 U, 3 P1, 4 P2) in the copies and in the FP32 factor
     auto hb = [&](int q, int j) -> void* {
       const size_t base = q < 3 ? q * N : 3 * N + (q - 3) * slots;
@@ -1351,8 +1371,18 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
     }
     const size_t ocs = pb.add_ptrs(cs), ocd = pb.add_ptrs(cd), occ = pb.add_ptrs(cc);
     pb.align();
-    TF_HIP(hipMalloc(&h->hpool, pb.bytes.size() + 64));
-    TF_HIP(tf::upload(h->hpool, pb.bytes.data(), pb.bytes.size()));
+    void* pool = nullptr;
+    e = hipMalloc(&pool, pb.bytes.size() + 64);
+    if (e == hipSuccess) e = tf::upload(pool, pb.bytes.data(), pb.bytes.size());
+    if (e != hipSuccess) {
+      (void)hipFree(pool);
+      (void)hipFree(H_);
+      (void)hipFree(S_);
+      return tf::fail_hip(e, "FP16 copy pointer lists");
+    }
+    h->H = H_;
+    h->S = S_;
+    h->hpool = pool;
     char* P = (char*)h->hpool;
     for (size_t l = 0; l < offs.size(); ++l) {
       BcrLevelDev& L = h->lev[l];
