@@ -22,7 +22,9 @@ Options (``parameters['HIP Solver']``, all optional):
                        converges quickly;
 ``'Inner Tolerance'``  (mixed) relative residual of each inner solve (1e-3: FP32
                        solves stall near 1e-5 on these Jacobians, so the FP64
-                       refinement does the rest);
+                       refinement does the rest); a step whose remaining
+                       reduction to ``'Tolerance'`` is at least 1e-5 asks for
+                       half of it instead, and so finishes the solve;
 ``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
 ``'Apply Graph'``      replay the line-factor apply as one captured hipGraph
                        (True);
@@ -70,6 +72,7 @@ from . import _lib
 from .vector import DeviceVector, ptr, stream, torch
 
 LINE_BUDGET_BYTES = 64 * 2 ** 30
+INNER_FLOOR = 1e-5    # FP32 inner solves stall near this relative residual on these Jacobians
 
 
 class SolverError(RuntimeError):
@@ -747,12 +750,19 @@ class Solver:
             else:
                 _lib.call('tf_f64_to_f32', n, ptr(b), ptr(r32), st)
             _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
+            # the last refinement step: when the remaining reduction is within what an
+            # FP32 inner solve reaches, ask for it (with margin) instead of 'Inner Tolerance'
+            itol = opts['Inner Tolerance']
+            if outer > 0:
+                need = 0.5 * opts['Tolerance'] / rel
+                if INNER_FLOOR <= need < itol:
+                    itol = need
             p_inner = inner_prec()
             if p_inner is prec and hasattr(prec, 'native'):
                 nk = self._krylov_for(opts['Restart'], 'float32', native=True)
-                its, est = nk.solve(jac, val32, g, prec, r32, d32, opts['Inner Tolerance'], inner_max, x_is_zero=True)
+                its, est = nk.solve(jac, val32, g, prec, r32, d32, itol, inner_max, x_is_zero=True)
             else:
-                its, est = kry.solve(matvec32, p_inner.apply32, r32, d32, opts['Inner Tolerance'], inner_max,
+                its, est = kry.solve(matvec32, p_inner.apply32, r32, d32, itol, inner_max,
                                      x_is_zero=True)
             self.trace.append((its, est, rel))
             total += its
