@@ -769,7 +769,10 @@ class Solver:
             _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)
             _lib.call('tf_axpby_f64', n, 1.0, ptr(x), 1.0, ptr(d), ptr(x), st)
             outer += 1
-            if not jac.prec_exact and its > opts['Reuse Limit']:
+            # the limit is for 'Inner Tolerance': a tighter last step may take
+            # proportionally more iterations without the factor being staler
+            limit = opts['Reuse Limit'] * math.log(itol) / math.log(opts['Inner Tolerance'])
+            if not jac.prec_exact and its > limit:
                 # lagged factor too stale: refactor this Jacobian and keep refining
                 # (in the background: with the lagged factor until the new one is ready)
                 if bg is not None:
