@@ -302,7 +302,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs the continuation hot path on a single MI355X '
-              '(multi-GPU sharding is described in DESIGN.md, not implemented); use --gpus 1', file=sys.stderr)
+              '(sharding over several GPUs is not implemented); use --gpus 1', file=sys.stderr)
         sys.exit(2)
 
     if args.kernel_sample:

@@ -1,7 +1,7 @@
 """Shared helpers for the parity tests (HIP backend vs the CPU oracle)."""
 import numpy
 
-REL_TOL = 1e-12   # north-star bound for FP64 values (relative to the largest entry)
+REL_TOL = 1e-12   # parity bound for FP64 values (relative to the largest entry)
 
 
 def csr_of_hip(jac):
