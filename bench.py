@@ -301,7 +301,7 @@ def main():
                     help='run only the short launch sequence used for hardware-counter profiles')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build runs the continuation hot path on a single MI355X '
+        print('bench.py measures one GPU: the continuation hot path runs on a single MI355X '
               '(sharding over several GPUs is not implemented); use --gpus 1', file=sys.stderr)
         sys.exit(2)
 
