@@ -1,8 +1,8 @@
 """CPU oracle of the learned preconditioner's convolutions (test infrastructure
 only; never called by the product path).
 
-The network has no reference counterpart: it is one of the build-only
-subsystems of the GPU backend.  These functions restate its arithmetic in
+The network has no reference counterpart: it exists only in the GPU
+backend.  These functions restate its arithmetic in
 numpy so that the BF16 MFMA kernels (transiflow_amd/csrc/conv.hip) can be
 checked: forward, the gradients with respect to weights, bias and input, and
 BF16 rounding.  Pinning (tests/test_learned_oracle.py): single-channel layers against
