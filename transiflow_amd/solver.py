@@ -20,7 +20,7 @@ Options (``parameters['HIP Solver']``, all optional):
                        values, FP32 preconditioner factor and apply); the FP32
                        factor is reused across Jacobians while the inner solve
                        converges quickly;
-``'Inner Tolerance'``  (mixed) relative residual of each inner solve (1e-3: FP32
+``'Inner Tolerance'``  (mixed) relative residual of each inner solve (5e-4: FP32
                        solves stall near 1e-5 on these Jacobians, so the FP64
                        refinement does the rest); a step whose remaining
                        reduction to ``'Tolerance'`` is at least 1e-5 asks for
@@ -42,7 +42,8 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Restart'``          Krylov basis size (30);
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
-                       preconditioner is refactored (5); a refinement step that
+                       preconditioner is refactored (8, for 'Inner Tolerance';
+                       scaled by the reduction a step asks for); a refinement step that
                        stagnates on a lagged factor also refactors;
 ``'Factor GEMM'``      (mixed, line factor) ``'fp32'`` (default), ``'bf16x3'`` or
                        ``'bf16'``: the level GEMMs of the FP32 factor on FP32
@@ -87,8 +88,8 @@ def _opts(parameters):
         'Tolerance': float(o.get('Tolerance', 1e-10)),
         'Restart': int(o.get('Restart', 30)),
         'Maximum Iterations': int(o.get('Maximum Iterations', 300)),
-        'Reuse Limit': int(o.get('Reuse Limit', 5)),
-        'Inner Tolerance': float(o.get('Inner Tolerance', 1e-3)),
+        'Reuse Limit': int(o.get('Reuse Limit', 8)),
+        'Inner Tolerance': float(o.get('Inner Tolerance', 5e-4)),
         'Inner Iterations': int(o.get('Inner Iterations', 10)),
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
         'Learned Network': o.get('Learned Network'),
