@@ -95,7 +95,8 @@ int tf_spmv_f64(int n, const int* row_ptr, const int* col, const double* val, co
                 double* y, int gauge, void* stream);
 int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, const float* x,
                 float* y, int gauge, void* stream);
-int tf_set_entry_f64(double* x, long i, double v, void* stream);
+/* x[i] = v for 0 <= i < n (an error otherwise). */
+int tf_set_entry_f64(double* x, long n, long i, double v, void* stream);
 
 /* C = alpha A + beta B for CSR matrices (scipy sparse addition, as used by
  * TimeIntegration._newton, TimeIntegration.py:56: jacobian(x) - mass / (theta dt)).

@@ -200,6 +200,31 @@ def test_solve_mixed_fallback(hip):
     assert itf.solver.fallbacks == 1
 
 
+def test_solve_mixed_stale_factor_refactors(hip):
+    """A lagged FP32 factor from a far-away Jacobian (Re = 0 at rest, used for
+    Re = 2000 at a moving state) with the reuse limit out of reach: the
+    refinement stagnates, the solver refactors that Jacobian and the solve
+    still meets 1e-10 and agrees with splu."""
+    from oracle.interface import Interface as OracleInterface
+    params = {'Reynolds Number': 0.0}
+    itf = hip.Interface(dict(params, **{'HIP Solver': {'Precision': 'mixed', 'Reuse Limit': 1000}}), 48, 40)
+    ref = OracleInterface(dict(params), 48, 40)
+    rhs = seeded_state(itf.n, 51)
+    itf.solve(itf.jacobian(itf.vector()), rhs)
+    assert itf.solver.stale_refactors == 0
+    itf.set_parameter('Reynolds Number', 2000.0)
+    ref.set_parameter('Reynolds Number', 2000.0)
+    state = seeded_state(itf.n, 52) * 0.3
+    y = itf.array_from_vector(itf.solve(itf.jacobian(state), rhs))
+    yo = ref.solve(ref.jacobian(state), rhs)
+    err = numpy.linalg.norm(y - yo) / numpy.linalg.norm(yo)
+    print('stale factor: refactors %d (stagnation) %d (limit), refinement trace %s, vs splu %.2e'
+          % (itf.solver.stale_refactors, itf.solver.limit_refactors, itf.solver.trace, err), flush=True)
+    assert itf.solver.stale_refactors == 1 and itf.solver.limit_refactors == 0
+    assert itf.solver.log[-1][1] <= 1e-10 and itf.solver.log[-1][3]
+    assert err <= 1e-8
+
+
 def test_newton_ldc2d_16(hip):
     """C1: Newton from zero at Re = 100 against the reference's result."""
     g = numpy.load(os.path.join(GOLDEN, 'ldc2d_16.npz'))

@@ -211,3 +211,18 @@ def test_backend_fails_loudly_without_gpu():
     from transiflow_amd._lib import BackendError
     with pytest.raises(BackendError):
         transiflow_amd.Interface({}, 8, 8)
+
+
+def test_solve_rejects_bad_rhs_shapes():
+    """Solver.solve accepts (n,) and (n, k) right-hand sides (SciPy.py:209-216)
+    and raises ValueError for anything else, before touching the device."""
+    from transiflow_amd.solver import Solver
+
+    class _Itf:
+        n = 12
+        parameters = {}
+
+    s = Solver(_Itf())
+    for bad in (numpy.zeros(11), numpy.zeros((11, 2)), numpy.zeros((12, 2, 1)), numpy.zeros(0)):
+        with pytest.raises(ValueError):
+            s.solve(None, bad)

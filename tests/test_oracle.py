@@ -18,6 +18,7 @@ import pytest
 from oracle import layout
 from oracle.crs import assemble_from_atom
 from oracle.discretization import Discretization
+from oracle.interface import Interface as OracleInterface
 from tests.golden.fixtures import read_matrix, read_vector
 
 GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
@@ -198,6 +199,64 @@ def test_coordinate_vectors():
     x = layout.stretched_coordinate_vector(0, 1, 5, 1.5)
     assert x[-1] == pytest.approx(0) and x[0] > 0 and x[4] == pytest.approx(1)
     assert x[1] - x[0] < x[2] - x[1]
+
+
+@pytest.mark.parametrize('n,sigma', [(5, 0.1), (16, 0.1), (33, 0.15)])
+def test_sin_stretched_coordinate_vector(n, sigma):
+    """utils.py:234-267 ('Grid Stretching Method': 'sin'), pinned by closed forms:
+    the faces are t - sigma sin(2 pi t) of the uniform faces t, so both ends stay
+    where they are, the map is point-symmetric about the middle (x(1-t) = 1 - x(t)),
+    monotone for sigma < 1/(2 pi) with the finest cells at the walls, and the two
+    ghost cells mirror the first and last cells."""
+    x = layout.stretched_coordinate_vector_sin(0, 1, n, sigma)
+    assert len(x) == n + 3
+    assert x[-1] == 0.0 and x[n - 1] == pytest.approx(1.0, abs=1e-15)
+    faces = x[:n - 1]                       # interior faces t = (i+1)/n, i < n-1
+    assert numpy.allclose(faces + faces[::-1], 1.0, atol=1e-15)
+    widths = numpy.diff(numpy.concatenate([[x[-1]], x[:n]]))
+    assert numpy.all(widths > 0)
+    assert widths[0] < widths[n // 2] and widths[-1] < widths[n // 2]
+    assert widths[0] == pytest.approx(widths[-1], rel=1e-12)
+    # slope 1 - 2 pi sigma at the walls: n times the first width tends to it, with
+    # the sine's cubic term, sigma (2 pi)^3 / (6 n^2), as the bound of the gap
+    gap = widths[0] * n - (1 - 2 * numpy.pi * sigma)
+    assert 0 < gap <= sigma * (2 * numpy.pi) ** 3 / (6 * n * n) * (1 + 1e-9)
+    assert x[-2] == pytest.approx(-x[0], abs=1e-15)            # ghost left of the start
+    assert x[n] == pytest.approx(2 * x[n - 1] - x[n - 2], abs=1e-15)  # ghost right of the end
+    # a domain [a, b] is the unit map scaled; only the ghost cell on a side
+    # that is not 0 (start) or 1 (end) is left as the uniform formula gives it
+    y = layout.stretched_coordinate_vector_sin(2.0, 5.0, n, sigma)
+    assert numpy.allclose(y[:n], 2.0 + 3.0 * x[:n], atol=1e-14)
+    assert y[-1] == 2.0
+
+
+def test_iterative_solve_velocity_recovery():
+    """tests/test_SciPy.py:32-67: the spilu-preconditioned GMRES solve at
+    tolerance 1e-8 recovers the velocity components of x from b = J x to
+    10 tol ||b||, while the pressure, fixed only up to the gauge, stays
+    inaccurate.  Also checked: the returned y meets the GMRES tolerance on the
+    gauged system, and the direct solve of the same system agrees with it."""
+    dim, dof, nx = 3, 4, 4
+    params = {'Use Iterative Solver': True, 'Iterative Solver': {'Convergence Tolerance': 1e-8}}
+    itf = OracleInterface(params, nx, nx, nx, dim, dof)
+    n = dof * nx ** 3
+    x = numpy.random.default_rng(7).random(n)
+    A = itf.jacobian(numpy.zeros(n))
+    b = A @ x
+    y = itf.solve(A, b)
+    tol = 1e-8 * numpy.linalg.norm(b) * 10
+    y[dim::dof] += x[dim]
+    assert numpy.linalg.norm(y - x) > tol / 10
+    for d in range(dim):
+        assert numpy.linalg.norm(y[d::dof] - x[d::dof]) < tol
+    G = itf.compute_bordered_matrix(A, fix_pressure_row=True)
+    bg = b.copy()
+    bg[itf.pressure_row] = 0
+    y = itf.solve(A, b)
+    assert numpy.linalg.norm(bg - G @ y) <= 1e-8 * numpy.linalg.norm(bg) * (1 + 1e-6)
+    yd = itf.direct_solve(A, b)
+    for d in range(dim):
+        assert numpy.linalg.norm(y[d::dof] - yd[d::dof]) < tol
 
 
 def test_state_layout_roundtrip():

@@ -15,6 +15,7 @@
 // CrsMatrix.compress (CrsMatrix.py:52-71).
 #include "stencil.h"
 #include "tf_common.h"
+#include "transiflow_hip.h"
 
 using namespace tf;
 
@@ -271,6 +272,22 @@ int scan_blocks(tf_problem* p, int* row_ptr, hipStream_t st) {
 
 }  // namespace
 
+// Device buffers of a problem; on failure the caller frees the partial set
+// (every pointer starts null, and hipFree/hipHostFree accept null).
+static int problem_alloc(tf_problem* p, const double* const host[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const size_t bytes = sizeof(double) * (p->n[a] + 3);
+    TF_HIP(hipMalloc(&p->coord_d[a], bytes));
+    TF_HIP(tf::upload(p->coord_d[a], host[a], bytes));
+  }
+  TF_HIP(hipMalloc(&p->ops_d, sizeof(BcOp) * MAX_BC_OPS));
+  TF_HIP(hipMalloc(&p->counts_d, sizeof(int) * p->nrow));
+  TF_HIP(hipMalloc(&p->block_sums_d, sizeof(int) * p->nblocks));
+  TF_HIP(hipMalloc(&p->block_off_d, sizeof(int) * p->nblocks));
+  TF_HIP(hipHostMalloc(&p->nnz_pinned, sizeof(int), hipHostMallocDefault));
+  return 0;
+}
+
 extern "C" {
 
 const char* tf_last_error(void) { return tf::last_error().c_str(); }
@@ -304,16 +321,10 @@ int tf_problem_create(tf_problem** out, int nx, int ny, int nz, int dim, int dof
   p->nrow = p->ncell * dof;
   p->nblocks = (p->ncell + 63) / 64;
   const double* host[3] = {x, y, z};
-  for (int a = 0; a < 3; ++a) {
-    const size_t bytes = sizeof(double) * (p->n[a] + 3);
-    TF_HIP(hipMalloc(&p->coord_d[a], bytes));
-    TF_HIP(tf::upload(p->coord_d[a], host[a], bytes));
+  if (int e = problem_alloc(p, host)) {
+    tf_problem_destroy(p);  // frees what was allocated before the failure
+    return e;
   }
-  TF_HIP(hipMalloc(&p->ops_d, sizeof(BcOp) * MAX_BC_OPS));
-  TF_HIP(hipMalloc(&p->counts_d, sizeof(int) * p->nrow));
-  TF_HIP(hipMalloc(&p->block_sums_d, sizeof(int) * p->nblocks));
-  TF_HIP(hipMalloc(&p->block_off_d, sizeof(int) * p->nblocks));
-  TF_HIP(hipHostMalloc(&p->nnz_pinned, sizeof(int), hipHostMallocDefault));
   p->ph = Physics{};
   p->ph.ops = p->ops_d;
   *out = p;

@@ -314,7 +314,8 @@ int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, doubl
   return 0;
 }
 
-int tf_set_entry_f64(double* x, long i, double v, void* stream) {
+int tf_set_entry_f64(double* x, long n, long i, double v, void* stream) {
+  if (i < 0 || i >= n) return tf::fail("tf_set_entry_f64: index outside the vector");
   hipLaunchKernelGGL(k_set_entry, dim3(1), dim3(1), 0, (hipStream_t)stream, x, i, v);
   TF_LAUNCH_CHECK("k_set_entry");
   return 0;

@@ -87,13 +87,20 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
   k->m = restart;
   k->single = single;
   k->es = single ? sizeof(float) : sizeof(double);
-  TF_HIP(hipMalloc(&k->V, k->es * (size_t)n * (restart + 1)));
-  TF_HIP(hipMalloc(&k->Z, k->es * (size_t)n * restart));
-  TF_HIP(hipMalloc(&k->w, k->es * (size_t)n));
-  TF_HIP(hipMalloc(&k->h, sizeof(double) * (restart + 2)));
-  TF_HIP(hipMalloc(&k->y, sizeof(double) * (restart + 1)));
-  TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
-  TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
+  auto alloc = [&]() -> int {
+    TF_HIP(hipMalloc(&k->V, k->es * (size_t)n * (restart + 1)));
+    TF_HIP(hipMalloc(&k->Z, k->es * (size_t)n * restart));
+    TF_HIP(hipMalloc(&k->w, k->es * (size_t)n));
+    TF_HIP(hipMalloc(&k->h, sizeof(double) * (restart + 2)));
+    TF_HIP(hipMalloc(&k->y, sizeof(double) * (restart + 1)));
+    TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
+    TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
+    return 0;
+  };
+  if (int e = alloc()) {
+    tf_krylov_destroy(k);  // frees the partial set (pointers start null)
+    return e;
+  }
   *out = k;
   return 0;
 }

@@ -24,6 +24,7 @@
 // (scipy.sparse.linalg.splu in transiflow/interface/SciPy.py:131-162,194-258)
 // for problems whose grid lines fit a dense block.
 #include "tf_common.h"
+#include "transiflow_hip.h"
 
 #include <vector>
 
@@ -507,21 +508,28 @@ int tf_linelu_create(tf_linelu** out, int n, int m, int single) {
   h->single = single;
   const size_t es = single ? sizeof(float) : sizeof(double);
   const size_t mm = (size_t)m * m;
-  TF_HIP(hipMalloc(&h->X, sizeof(double) * mm));
-  TF_HIP(hipMalloc(&h->Y, es * mm * h->N));
-  TF_HIP(hipMalloc(&h->G, es * mm * h->N));
-  TF_HIP(hipMalloc(&h->W, es * mm * h->N));
-  TF_HIP(hipMalloc(&h->Lc, sizeof(Coupling) * KC * (size_t)n));
-  TF_HIP(hipMalloc(&h->Ut, sizeof(Coupling) * KC * (size_t)n));
-  TF_HIP(hipMalloc(&h->lcount, sizeof(int) * n));
-  TF_HIP(hipMalloc(&h->ucount, sizeof(int) * n));
-  TF_HIP(hipMalloc(&h->err, sizeof(int) * 2));
-  h->nperturb = h->err + 1;
-  TF_HIP(hipMalloc(&h->Pinv, sizeof(double) * PB * PB));
-  TF_HIP(hipMalloc(&h->Tm, sizeof(double) * PB * (size_t)m));
-  TF_HIP(hipMalloc(&h->Uc, sizeof(double) * PB * (size_t)m));
-  TF_HIP(hipMalloc(&h->ztmp, sizeof(double) * (size_t)n));
-  TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
+  auto alloc = [&]() -> int {
+    TF_HIP(hipMalloc(&h->X, sizeof(double) * mm));
+    TF_HIP(hipMalloc(&h->Y, es * mm * h->N));
+    TF_HIP(hipMalloc(&h->G, es * mm * h->N));
+    TF_HIP(hipMalloc(&h->W, es * mm * h->N));
+    TF_HIP(hipMalloc(&h->Lc, sizeof(Coupling) * KC * (size_t)n));
+    TF_HIP(hipMalloc(&h->Ut, sizeof(Coupling) * KC * (size_t)n));
+    TF_HIP(hipMalloc(&h->lcount, sizeof(int) * n));
+    TF_HIP(hipMalloc(&h->ucount, sizeof(int) * n));
+    TF_HIP(hipMalloc(&h->err, sizeof(int) * 2));
+    h->nperturb = h->err + 1;
+    TF_HIP(hipMalloc(&h->Pinv, sizeof(double) * PB * PB));
+    TF_HIP(hipMalloc(&h->Tm, sizeof(double) * PB * (size_t)m));
+    TF_HIP(hipMalloc(&h->Uc, sizeof(double) * PB * (size_t)m));
+    TF_HIP(hipMalloc(&h->ztmp, sizeof(double) * (size_t)n));
+    TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
+    return 0;
+  };
+  if (int e = alloc()) {
+    tf_linelu_destroy(h);  // frees the partial set (pointers start null)
+    return e;
+  }
   *out = h;
   return 0;
 }

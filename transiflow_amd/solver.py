@@ -515,6 +515,8 @@ class Solver:
         self.log = []
         self.refinements = []  # (mixed) outer refinement steps per solve
         self.fallbacks = 0     # (mixed) solves finished in FP64
+        self.stale_refactors = 0   # (mixed) lagged factors replaced because refinement stagnated
+        self.limit_refactors = 0   # (mixed) ... because an inner solve exceeded the reuse limit
         self.trace = []        # (mixed) per refinement step: (inner its, inner estimate, outer FP64 rel)
         self.observer = None   # optional callable(jac, precond32, v) on every inner preconditioner input
         self._learned = None
@@ -602,10 +604,27 @@ class Solver:
         return jac.prec, True
 
     def solve(self, jac, rhs, rhs2=None, V=None, W=None, C=None):
+        """y with (gauged J) y = rhs.  ``rhs`` of length n gives a DeviceVector;
+        an (n, k) array gives an (n, k) numpy array whose columns are solved
+        one after another, each with its ``pressure_row`` entry zeroed
+        (SciPy.py:209-216, where splu solves all columns of the gauged rhs)."""
         if rhs2 is not None or V is not None:
             return self.solve_bordered(jac, rhs, rhs2, V, W, C)
-        b_vec = rhs if isinstance(rhs, DeviceVector) else DeviceVector.from_array(rhs)
-        return DeviceVector(self._solve_device(jac, b_vec.data, zero_gauge=True))
+        n = self.itf.n
+        if isinstance(rhs, DeviceVector):
+            if rhs.size != n:
+                raise ValueError('rhs has %d entries, the system has %d rows' % (rhs.size, n))
+            return DeviceVector(self._solve_device(jac, rhs.data, zero_gauge=True))
+        a = numpy.asarray(rhs, dtype=numpy.float64)
+        if a.ndim == 1 and a.shape[0] == n:
+            return DeviceVector(self._solve_device(jac, DeviceVector.from_array(a).data, zero_gauge=True))
+        if a.ndim == 2 and a.shape[0] == n:
+            out = numpy.empty_like(a)
+            for j in range(a.shape[1]):
+                col = DeviceVector.from_array(a[:, j]).data
+                out[:, j] = self._solve_device(jac, col, zero_gauge=True).cpu().numpy()
+            return out
+        raise ValueError('rhs must have shape (%d,) or (%d, k), got %s' % (n, n, a.shape))
 
     def _solve_device(self, jac, rhs, zero_gauge):
         """x with (gauged J) x = rhs; ``zero_gauge`` zeroes rhs[pressure_row] first
@@ -613,9 +632,12 @@ class Solver:
         opts = _opts(self.itf.parameters)
         T = torch()
         g = self.gauge()
+        if rhs.dim() != 1 or rhs.numel() != self.itf.n or rhs.dtype != T.float64:
+            raise ValueError('right-hand side of shape %s (%s) for a system of %d FP64 rows'
+                             % (tuple(rhs.shape), rhs.dtype, self.itf.n))
         b = rhs.clone()
         if g >= 0 and zero_gauge:
-            _lib.call('tf_set_entry_f64', ptr(b), g, 0.0, stream())
+            _lib.call('tf_set_entry_f64', ptr(b), b.numel(), g, 0.0, stream())
         x = T.empty_like(b)
         _lib.call('tf_fill_f64', x.numel(), 0.0, ptr(x), stream())
         if opts['Precision'] == 'mixed':
@@ -736,6 +758,7 @@ class Solver:
                     prec = jac.prec
                 if rel > 0.5 * prev and not jac.prec_exact:
                     # the lagged factor no longer contracts: factor this Jacobian
+                    self.stale_refactors += 1
                     prec, fresh = self._refactor_now(jac, opts)
                     prev = math.inf
                 elif rel > 0.5 * prev and jac.prec_exact:
@@ -776,6 +799,7 @@ class Solver:
             if not jac.prec_exact and its > limit:
                 # lagged factor too stale: refactor this Jacobian and keep refining
                 # (in the background: with the lagged factor until the new one is ready)
+                self.limit_refactors += 1
                 if bg is not None:
                     bg.start(jac)
                 else:
@@ -837,7 +861,7 @@ class Solver:
         for w in Wc:
             wt = w.clone()
             if g >= 0:
-                _lib.call('tf_set_entry_f64', ptr(wt), g, 0.0, stream())
+                _lib.call('tf_set_entry_f64', ptr(wt), wt.numel(), g, 0.0, stream())
             Wt.append(DeviceVector(wt))
         S = s * s * Cm - s * numpy.array([[Wt[i].dot(DeviceVector(Bc[j])) for j in range(k)] for i in range(k)])
         rr = s * r2 - s * numpy.array([Wt[i].dot(DeviceVector(a)) for i in range(k)])
