@@ -105,6 +105,15 @@ int tf_csr_add_count(int n, const int* ra, const int* ca, const int* rb, const i
                      int* nnz_host, void* stream);
 int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, double alpha, const int* rb,
                     const int* cb, const double* vb, double beta, const int* rc, int* cc, double* vc, void* stream);
+/* Theta-method step residual, divided by theta dt (TimeIntegration._newton,
+ * TimeIntegration.py:47-49: M (u - v) + dt theta F(v) + dt (1 - theta) F(u)):
+ * out = Fv + cb Fu + cm mdiag .* (u - v) with cb = (1 - theta) / theta, cm = 1 / (theta dt)
+ * and mdiag the diagonal of the mass matrix. */
+int tf_theta_residual(long n, const double* Fv, const double* Fu, const double* mdiag, const double* u,
+                      const double* v, double cb, double cm, double* out, void* stream);
+/* diag[r] = A[r][r] of a CSR matrix, 0 for rows without a diagonal entry (the mass
+ * matrix of Discretization.mass_matrix, Discretization.py:417-437, is diagonal). */
+int tf_csr_diagonal(int n, const int* row_ptr, const int* col, const double* val, double* diag, void* stream);
 int tf_fill_f64(long n, double v, double* x, void* stream);
 /* FP32 vectors of the mixed-precision inner solve (no reference counterpart). */
 int tf_fill_f32(long n, double v, float* x, void* stream);

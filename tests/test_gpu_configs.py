@@ -43,6 +43,7 @@ def _continuation(hip, itf, x0_host, param, target, ds):
     return itf.array_from_vector(x), mu, numpy.array(mus), numpy.array(its)
 
 
+@pytest.mark.timeout(300)
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_cavity_continuation_256_matches_oracle(hip, precision):
     g = numpy.load(os.path.join(GOLDEN, 'cavity_continuation_256.npz'))
@@ -59,6 +60,7 @@ def test_cavity_continuation_256_matches_oracle(hip, precision):
     assert err <= 1e-8
 
 
+@pytest.mark.timeout(300)
 def test_rb_512x128_assembly_and_solve(hip):
     from oracle.interface import Interface as OracleInterface
     params = dict(RB, **{'Rayleigh Number': 5000.0})
@@ -84,6 +86,7 @@ def test_rb_512x128_assembly_and_solve(hip):
     assert res <= 1e-9 and err <= 1e-8
 
 
+@pytest.mark.timeout(400)
 def test_rb_512x128_continuation_matches_oracle(hip):
     """The configured Rayleigh continuation: Newton at Ra = 0 from the seeded
     1e-3 perturbation of T (linear, so it lands on the conduction state), then
@@ -96,7 +99,7 @@ def test_rb_512x128_continuation_matches_oracle(hip):
     h[itf.dof - 1::itf.dof] = 1e-3 * numpy.random.default_rng(1).uniform(-1, 1, itf.n // itf.dof)
     x, mu, mus, its = _continuation(hip, itf, h, 'Rayleigh Number', 5000, 100)
     ref = OracleInterface(dict(params), 512, 128)
-    cont = Continuation(ref, newton_tolerance=1e-8, printer=None)
+    cont = Continuation(ref, newton_tolerance=1e-8, printer=print)   # progress lines: the CPU run takes a minute
     rmus, rits = [], []
 
     def cb(_i, _x, m):
@@ -112,6 +115,7 @@ def test_rb_512x128_continuation_matches_oracle(hip):
     assert err <= 1e-8
 
 
+@pytest.mark.timeout(400)
 def test_batch_64_instances(hip):
     from transiflow_amd.batch import c4_instances, run_batch
     inst = c4_instances()

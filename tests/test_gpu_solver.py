@@ -362,20 +362,31 @@ def test_csr_combination_matches_scipy(hip):
 
 @pytest.mark.parametrize('theta', [1.0, 0.5])
 def test_time_integration_matches_oracle(hip, theta):
-    """Theta method (TimeIntegration.py:38-113) on the HIP backend against the same
-    driver on the oracle backend: LDC 16x16, Re 100, from rest, dt 0.05, 4 steps."""
+    """Theta method (contract TimeIntegration.py:40-113) on the HIP backend against
+    the oracle's theta method: LDC 16x16, Re 100, from rest, dt 0.05 to t = 0.2;
+    the same Newton solves per step and the same state to 1e-8."""
     from oracle.interface import Interface as OracleInterface
-    out = {}
-    for name, itf in (('hip', hip.Interface({'Reynolds Number': 100}, 16, 16)),
-                      ('oracle', OracleInterface({'Reynolds Number': 100}, 16, 16, 1, 2, 3))):
-        ti = hip.TimeIntegration(itf, theta=theta, newton_tolerance=1e-9, printer=None)
-        x, t = ti.integration(itf.vector(), 0.05, 0.2)
-        out[name] = (numpy.asarray(itf.array_from_vector(x) if name == 'hip' else x), t, ti.newton_iterations)
-    (xh, th, ih), (xo, to, io) = out['hip'], out['oracle']
+    from oracle.time_integration import TimeIntegration as OracleTI
+    itf = hip.Interface({'Reynolds Number': 100}, 16, 16)
+    ti = hip.TimeIntegration(itf, theta=theta, newton_tolerance=1e-9, printer=None)
+    seen = []
+    x, t = ti.integration(itf.vector(), 0.05, 0.2, callback=lambda _i, _x, tt: seen.append(tt))
+    ref = OracleInterface({'Reynolds Number': 100}, 16, 16, 1, 2, 3)
+    to = OracleTI(ref, theta=theta, newton_tolerance=1e-9)
+    xo, tr = to.integration(ref.vector(), 0.05, 0.2)
+    xh = itf.array_from_vector(x)
     err = numpy.linalg.norm(xh - xo) / numpy.linalg.norm(xo)
-    print('theta %.1f: steps %d, Newton its %s / %s, rel err %.2e' % (theta, len(ih), ih, io, err))
-    assert th == to and ih == io
+    print('theta %.1f: steps %d, Newton solves %s / %s, rel err %.2e' % (theta, len(ti.newton_iterations),
+                                                                          ti.newton_iterations, to.newton_iterations,
+                                                                          err))
+    assert t == tr and seen == [0, 0.05, 0.1, 0.15000000000000002, 0.2]
+    assert ti.newton_iterations == to.newton_iterations
     assert err <= 1e-8
+    # the step-size residual test ('dx') converges to the same state
+    ti2 = hip.TimeIntegration(hip.Interface({'Reynolds Number': 100}, 16, 16), theta=theta,
+                              newton_tolerance=1e-9, residual_check='dx', printer=None)
+    x2, _ = ti2.integration(ti2.interface.vector(), 0.05, 0.2)
+    assert numpy.linalg.norm(ti2.interface.array_from_vector(x2) - xo) <= 1e-8 * numpy.linalg.norm(xo)
 
 
 def test_checkpoint_resume(hip, tmp_path):

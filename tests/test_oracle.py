@@ -278,3 +278,48 @@ def test_mass_matrix_closed_form():
     assert M.coA[M.begA[row]] == pytest.approx(vol_u)
     prow = row + 3
     assert M.begA[prow + 1] == M.begA[prow]
+
+
+# -- theta-method time integration (oracle/time_integration.py) ----------------
+
+def _ti(params, nx, theta, tol=1e-11):
+    from oracle.time_integration import TimeIntegration
+    itf = OracleInterface(dict(params), nx, nx, 1, 2, 3)
+    return itf, TimeIntegration(itf, theta=theta, newton_tolerance=tol)
+
+
+def test_theta_steady_state_is_fixed_point():
+    """A state with F(x) = 0 is left unchanged by a step (no Newton solve needed)."""
+    from transiflow_amd.continuation import Continuation
+    itf, ti = _ti({'Reynolds Number': 100}, 8, 0.5, tol=1e-9)
+    xs = Continuation(itf, printer=None).newton(itf.vector(), tol=1e-12)
+    v = ti.step(xs, 0.3)
+    assert ti.newton_iterations == [0] and numpy.array_equal(v, xs)
+
+
+@pytest.mark.parametrize('theta,order', [(1.0, 1.0), (0.5, 2.0)])
+def test_theta_convergence_order(theta, order):
+    """Stokes cavity (Re = 0, linear F) from rest to t = 0.05: the velocity's
+    step-halving differences shrink by 2^order (backward Euler first order,
+    Crank-Nicolson second); a dropped or mis-weighted F(u) term breaks both."""
+    itf, _ = _ti({'Reynolds Number': 0}, 8, theta)
+    vel = numpy.arange(itf.vector().size) % 3 != 2
+    sols = []
+    for dt in (0.01, 0.005, 0.0025, 0.00125):
+        _, ti = _ti({'Reynolds Number': 0}, 8, theta)
+        x, t = ti.integration(itf.vector(), dt, 0.05 - 1e-9)
+        assert t == pytest.approx(0.05)
+        sols.append(x[vel])
+    d = [numpy.linalg.norm(sols[i] - sols[i + 1]) for i in range(3)]
+    rates = numpy.log2(numpy.array(d[:-1]) / numpy.array(d[1:]))
+    assert numpy.all(numpy.abs(rates - order) < 0.1), rates
+
+
+def test_theta_reaches_steady_state():
+    """Backward Euler with large steps ends at the steady Newton solution."""
+    from transiflow_amd.continuation import Continuation
+    itf, ti = _ti({'Reynolds Number': 100}, 8, 1.0, tol=1e-10)
+    xs = Continuation(itf, printer=None).newton(itf.vector(), tol=1e-12)
+    x, _ = ti.integration(itf.vector(), 2.0, 60.0)
+    vel = numpy.arange(x.size) % 3 != 2
+    assert numpy.linalg.norm(x[vel] - xs[vel]) <= 1e-8 * numpy.linalg.norm(xs[vel])

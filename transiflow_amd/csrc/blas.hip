@@ -110,6 +110,28 @@ __global__ __launch_bounds__(256) void k_div(long n, const double* x, double a, 
     out[i] = x[i] / a;
 }
 
+// Residual of one theta-method step from u to v, divided by theta dt:
+// G = F(v) + cb F(u) + cm m .* (u - v), m the diagonal of the (diagonal) mass matrix.
+__global__ __launch_bounds__(256) void k_theta_residual(long n, const double* __restrict__ Fv,
+                                                        const double* __restrict__ Fu, const double* __restrict__ m,
+                                                        const double* __restrict__ u, const double* __restrict__ v,
+                                                        double cb, double cm, double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = Fv[i] + cb * Fu[i] + cm * (m[i] * (u[i] - v[i]));
+}
+
+// diag[r] = A[r][r] (0 when the row holds no diagonal entry)
+__global__ __launch_bounds__(256) void k_csr_diagonal(int n, const int* __restrict__ row_ptr,
+                                                      const int* __restrict__ col, const double* __restrict__ val,
+                                                      double* __restrict__ diag) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 0.0;
+  for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+    if (col[p] == r) d += val[p];
+  diag[r] = d;
+}
+
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(256) void k_convert(long n, const Tin* __restrict__ x, Tout* __restrict__ out) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -311,6 +333,23 @@ int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, doubl
   hipLaunchKernelGGL(k_csr_add_fill, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ra, ca, va, alpha,
                      rb, cb, vb, beta, rc, cc, vc);
   TF_LAUNCH_CHECK("csr add fill");
+  return 0;
+}
+
+int tf_theta_residual(long n, const double* Fv, const double* Fu, const double* mdiag, const double* u,
+                      const double* v, double cb, double cm, double* out, void* stream) {
+  if (n < 1) return tf::fail("tf_theta_residual: empty vector");
+  hipLaunchKernelGGL(k_theta_residual, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, Fv, Fu, mdiag, u, v,
+                     cb, cm, out);
+  TF_LAUNCH_CHECK("k_theta_residual");
+  return 0;
+}
+
+int tf_csr_diagonal(int n, const int* row_ptr, const int* col, const double* val, double* diag, void* stream) {
+  if (n < 1) return tf::fail("tf_csr_diagonal: empty matrix");
+  hipLaunchKernelGGL(k_csr_diagonal, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, row_ptr, col, val,
+                     diag);
+  TF_LAUNCH_CHECK("k_csr_diagonal");
   return 0;
 }
 
