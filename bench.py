@@ -9,13 +9,20 @@ One "step" is one accepted pseudo-arclength step; the timed region is the
 whole continuation including the initial Newton solve and tangent, with the
 state resident in HBM.
 
-Variants: ``fp64`` (FGMRES to a 1e-10 relative residual with the exact FP64
-line factorisation, refactored every Newton iteration) and ``mixed`` (FP64
+Variants: ``fp64`` (the plain FP64 method: FGMRES to a 1e-10 relative
+residual with the exact FP64 line factorisation, refactored for every
+Jacobian as the reference refactors its splu), ``fp64reuse`` (the same with
+the factor of an earlier Jacobian reused while the solves stay short: what
+factor lagging alone buys, without any reduced precision), ``mixed16`` (FP64
 iterative refinement -- residual, update and convergence test in FP64 --
-around FP32 inner FGMRES with an FP32 line factorisation that is reused across
-Newton iterations and steps while the inner solves converge quickly).
-``value`` is the fastest variant that reproduces the FP64 variant's final
-state to 1e-8.
+around FP32 inner FGMRES with an FP32 line factorisation, reused across
+Jacobians, whose apply reads FP16 copies of its blocks) and ``learned16``
+(mixed16 with the learned convolutional correction in the inner
+preconditioner; the network is trained on the GPU before the timed runs on
+triples recorded from a 48 x 48 cavity continuation, and the correction
+judges itself and switches off when it stops paying).  ``value`` is the
+learned16 rate when it reproduces the FP64 variant's final state to 1e-8 (the
+metric's "learned+mixed"), else the fastest variant that does.
 
 The roofline object describes the kernel with the largest summed time in a
 profiled repeat of the headline variant (HIP events around each launch of the
@@ -57,11 +64,12 @@ def workload(name, n, target, ds):
                 'shape': (512, 128, 1), 'param': 'Rayleigh Number', 'target': 5000.0, 'ds': 100.0, 'perturb': True,
                 'label': 'Rayleigh-Benard 2D 512x128 (n=262144), Pr=1, Biot=1, X-max=4, T perturbed by '
                          '1e-3*U(-1,1) (default_rng(1)), Newton at Ra=0, continuation Ra 0->5000, ds=100, tol 1e-8'}
-    if name == 'ldc3d':
-        return {'params': {}, 'shape': (32, 32, 32), 'param': 'Reynolds Number', 'target': 1000.0, 'ds': 100.0,
+    if name in ('ldc3d', 'ldc3d64'):
+        k = 32 if name == 'ldc3d' else 64
+        return {'params': {}, 'shape': (k, k, k), 'param': 'Reynolds Number', 'target': 1000.0, 'ds': 100.0,
                 'perturb': False,
-                'label': 'lid-driven cavity 3D 32^3 (n=131072), Newton from zero at Re=0, continuation Re 0->1000, '
-                         'ds=100, tol 1e-8'}
+                'label': 'lid-driven cavity 3D %d^3 (n=%d), Newton from zero at Re=0, continuation Re 0->1000, '
+                         'ds=100, tol 1e-8' % (k, 4 * k ** 3)}
     return {'params': {}, 'shape': (n, n, 1), 'param': 'Reynolds Number', 'target': target, 'ds': ds,
             'perturb': False,
             'label': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength continuation '
@@ -72,6 +80,8 @@ def workload(name, n, target, ds):
 # applied from its FP32 blocks, and from FP16 copies of them
 VARIANTS = {
     'fp64': {'Precision': 'fp64'},
+    'fp64reuse': {'Precision': 'fp64', 'Reuse Factor': True},
+    'learned16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},   # + the trained network
     'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
     'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
     'mixed16bg': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True},
@@ -80,6 +90,9 @@ VARIANTS = {
 }
 DTYPES = {
     'fp64': 'f64',
+    'fp64reuse': 'f64 (factor of an earlier Jacobian reused while the solves stay short)',
+    'learned16': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies '
+                 'of the factor blocks with f32 row scales) + bf16 MFMA network correction (judged, f32 accumulate)',
     'mixed': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
     'mixed16': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies '
                'of the factor blocks with f32 row scales)',
@@ -92,10 +105,42 @@ DTYPES = {
 }
 
 
+LEARNED = {}   # the trained network of the learned16 variant (made once per process)
+
+
+def train_learned(tf, torch, n=48, target=1000.0):
+    """Train the learned correction as the A/B test does: triples (Jacobian,
+    residual left by the FP32 line factor, FP64 correction) recorded from the
+    mixed continuation of an n x n cavity (Re 0 -> target), 400 Adam steps of
+    the BF16 MFMA network on 70% of them; the held-out 30% measure the gain."""
+    from transiflow_amd import learned
+    t0 = time.perf_counter()
+
+    def run(itf):
+        cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
+        cont.continuation(cont.newton(itf.vector(), tol=1e-8), 'Reynolds Number', 0, target, 100)
+
+    itf, recs = learned.collect(lambda: tf.Interface({'HIP Solver': {'Precision': 'mixed'}}, n, n), run,
+                                every=2, max_samples=120)
+    pairs = learned.make_pairs(itf, recs)
+    ntr = int(0.7 * len(pairs))
+    net = learned.ConvNet(2, 3, seed=0)
+    hist = learned.train(net, pairs[:ntr], (n, n, 1), steps=400, lr=1e-3)
+    torch.cuda.synchronize()
+    info = {'grid': '%dx%d cavity, Re 0->%g' % (n, n, target), 'triples': len(pairs), 'train_steps': 400,
+            'loss_first20': float(sum(hist[:20]) / 20), 'loss_last20': float(sum(hist[-20:]) / 20),
+            'heldout_gain_fraction_of_r1sq': learned.evaluate(net, itf, recs[ntr:]),
+            'seconds': time.perf_counter() - t0}
+    LEARNED['net'], LEARNED['info'] = net, info
+    return net
+
+
 def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     import numpy
     params = dict(spec['params'])
     params['HIP Solver'] = dict(VARIANTS[variant], Restart=restart)
+    if variant == 'learned16':
+        params['HIP Solver']['Learned Network'] = LEARNED['net']
     itf = tf.Interface(params, *spec['shape'])
     cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
     start = itf.vector()
@@ -104,24 +149,33 @@ def run_continuation(tf, torch, spec, variant, maxit, restart=30):
         h[itf.dof - 1::itf.dof] = 1e-3 * numpy.random.default_rng(1).uniform(-1, 1, itf.n // itf.dof)
         start = itf.vector_from_array(h)
     torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
     t0 = time.perf_counter()
     x0 = cont.newton(start, tol=1e-8)
     newton0 = cont.newton_iterations
     x, mu = cont.continuation(x0, spec['param'], 0, spec['target'], spec['ds'], maxit=maxit)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    free1, _ = torch.cuda.mem_get_info()
     acc = sum(1 for h in cont.history if h[0] == 'accepted')
     iters = sum(h[1] for h in cont.history)
     fact = sum(1 for entry in itf.solver.log if entry[3])
     gm = sum(entry[0] for entry in itf.solver.log)
     worst = max(entry[1] for entry in itf.solver.log)
     bg = itf.solver._bg
+    lc = itf.solver._learned
     return {
         'itf': itf, 'x': x, 'mu': mu, 'seconds': dt, 'accepted': acc, 'corrector_iterations': iters,
         'initial_newton_iterations': newton0, 'solves': len(itf.solver.log), 'factorizations': fact,
         'gmres_iterations': gm, 'worst_relative_residual': worst,
         'background_factorizations': bg.started if bg is not None else 0,
         'steps_per_s': acc / dt,
+        'learned': ({'applied': lc.applied, 'accepted': lc.accepted, 'switched_off': lc.off,
+                     'mean_gain_when_accepted': lc.gain_sum / lc.accepted if lc.accepted else 0.0}
+                    if lc is not None else None),
+        'preconditioner': itf.solver.log[-1][2] if itf.solver.log else None,
+        'device_memory_GB': {'held_by_run': (free0 - free1) / 1e9, 'in_use_after': (total - free1) / 1e9,
+                             'total': total / 1e9},
     }
 
 
@@ -192,10 +246,66 @@ def profiled_run(tf, torch, spec, precision, maxit):
     return r, out
 
 
+def vector_rooflines(torch, itf, x, reps=50, j=15):
+    """Event-timed SpMV (FP64 and FP32 values, pressure gauge applied) on the
+    Jacobian at x, and the vector work of one FP32 CGS2 Arnoldi step at basis
+    size j+1 (two multidot / multiaxpy passes and the normalisation), against
+    their algorithmic bytes: SpMV 12 nnz + 4(n+1) + 16 n (FP64),
+    8 nnz + 4(n+1) + 8 n (FP32); Arnoldi 4 n (2 (2 (j+1) + 3) + 2)."""
+    from transiflow_amd import _lib
+    from transiflow_amd.vector import ptr, stream
+    J = itf.jacobian(x)
+    n, nnz, g = itf.n, J.nnz, itf.solver.gauge()
+    val32 = J.values32()
+    v64 = torch.rand(n, dtype=torch.float64, device='cuda')
+    y64 = torch.empty_like(v64)
+    v32, y32 = v64.float(), torch.empty(n, dtype=torch.float32, device='cuda')
+    V = torch.rand((j + 2, n), dtype=torch.float32, device='cuda')
+    w = torch.rand(n, dtype=torch.float32, device='cuda')
+    work = torch.empty(_lib.load().tf_krylov_work_size(), dtype=torch.float64, device='cuda')
+    h = torch.zeros(j + 3, dtype=torch.float64, device='cuda')
+    h2 = torch.zeros(j + 3, dtype=torch.float64, device='cuda')
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    st = stream()
+    t64 = timed(lambda: _lib.call('tf_spmv_f64', n, ptr(J.row_ptr), ptr(J.col), ptr(J.val), ptr(v64), ptr(y64), g, st))
+    t32 = timed(lambda: _lib.call('tf_spmv_f32', n, ptr(J.row_ptr), ptr(J.col), ptr(val32), ptr(v32), ptr(y32), g, st))
+
+    def arnoldi():
+        for hh in (h, h2):
+            _lib.call('tf_multidot_f32', n, j + 1, ptr(V), n, ptr(w), ptr(work), ptr(hh), 0, st)
+            _lib.call('tf_multiaxpy_f32', n, j + 1, ptr(V), n, ptr(hh), ptr(w), -1.0, st)
+        _lib.call('tf_normalize_f32', n, ptr(w), ptr(work), ctypes.c_void_p(h.data_ptr() + 8 * (j + 1)),
+                  ptr(V[j + 1]), st)
+    ta = timed(arnoldi)
+    b64 = 12 * nnz + 4 * (n + 1) + 16 * n
+    b32 = 8 * nnz + 4 * (n + 1) + 8 * n
+    ba = 4 * n * (2 * (2 * (j + 1) + 3) + 2)
+    return {
+        'spmv_f64': {'bytes': b64, 'us': t64 * 1e6, 'GBps': b64 / t64 / 1e9, 'frac_hbm_peak': b64 / t64 / 1e9 / HBM_PEAK_GBS},
+        'spmv_f32': {'bytes': b32, 'us': t32 * 1e6, 'GBps': b32 / t32 / 1e9, 'frac_hbm_peak': b32 / t32 / 1e9 / HBM_PEAK_GBS},
+        'gmres_arnoldi_vector_ops_f32': {'basis': j + 1, 'bytes': ba, 'us': ta * 1e6, 'GBps': ba / ta / 1e9,
+                                         'frac_hbm_peak': ba / ta / 1e9 / HBM_PEAK_GBS},
+        'n': n, 'nnz': nnz,
+        'note': 'working sets of this size sit in the 256 MB Infinity Cache; rates are algorithmic bytes / time',
+    }
+
+
 def cpu_baseline(n, x_host, mu, iters_total, accepted, reps=3):
     """Time the oracle (numpy/scipy restatement of the SciPy backend) for
     ``reps`` Newton-corrector iterations at the final state and scale the mean
-    by the GPU run's iteration count (same algorithm, same counts)."""
+    by the GPU run's iteration count (same algorithm, same counts).  This is an
+    extrapolation, not a timed continuation: the Re = 0 iterations of the real
+    run are linear and cheaper, so it somewhat understates the CPU rate."""
     import numpy
     from oracle.interface import Interface as OracleInterface
     ref = OracleInterface({'Reynolds Number': mu}, n, n)
@@ -213,7 +323,7 @@ def cpu_baseline(n, x_host, mu, iters_total, accepted, reps=3):
     t_iter = (time.perf_counter() - t0) / reps
     est = t_iter * iters_total
     return {
-        'value': accepted / est, 'unit': 'steps/s', 'cores': 1, 'kind': 'port',
+        'value': accepted / est, 'unit': 'steps/s', 'cores': 1, 'kind': 'port', 'extrapolated': True,
         'sample': ('%d Newton-corrector iterations (each 2 rhs, 1 Jacobian, sparse LU via scipy splu, 2 solves) '
                    'of the oracle at %dx%d, Re=%.0f, %.2f s each on %s (os.cpu_count()=%s, SuperLU single-threaded); '
                    'scaled by the %d corrector/Newton iterations the same algorithm needed for %d accepted steps'
@@ -290,12 +400,14 @@ def main():
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--target', type=float, default=2000.0)
     ap.add_argument('--ds', type=float, default=100.0)
-    ap.add_argument('--variants', default='fp64,mixed,mixed16')
+    ap.add_argument('--variants', default=None,
+                    help="comma-separated solver variants (default: 'fp64,fp64reuse,mixed16,learned16' for the 2D "
+                         "cavity, 'fp64,mixed16' for the other workloads)")
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch', 'rb', 'ldc3d'],
+    ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch', 'rb', 'ldc3d', 'ldc3d64'],
                     help="'ldc': one 256x256 cavity continuation (default); 'batch': 64 seeded cavity "
                          "instances of uneven size and target Reynolds number; 'rb': Rayleigh-Benard 2D "
-                         "512x128; 'ldc3d': 3D cavity 32^3")
+                         "512x128; 'ldc3d': 3D cavity 32^3; 'ldc3d64': 3D cavity 64^3 (two-grid preconditioner)")
     ap.add_argument('--workers', type=int, default=8, help='batch workload: concurrent worker streams')
     ap.add_argument('--kernel-sample', action='store_true',
                     help='run only the short launch sequence used for hardware-counter profiles')
@@ -318,6 +430,12 @@ def main():
 
     maxit = args.steps if args.steps > 0 else 1000
     spec = workload(args.workload, args.n, args.target, args.ds)
+    if args.variants is None:
+        args.variants = 'fp64,fp64reuse,mixed16,learned16' if args.workload == 'ldc' else 'fp64,mixed16'
+    if 'learned16' in args.variants.split(','):
+        log('training the learned correction (untimed set-up)')
+        train_learned(tf, torch)
+        log('  %s' % LEARNED['info'])
     if args.warmup > 0:
         for var in args.variants.split(','):
             # also fills the GEMM tile tuner for this precision's shapes
@@ -340,16 +458,17 @@ def main():
             xv = r['x'].numpy()
             agree[var] = float(numpy.linalg.norm(xv - xref) / numpy.linalg.norm(xref))
     ok = [v for v in results if v == 'fp64' or agree.get(v, 1.0) <= 1e-8]
-    head = max(ok, key=lambda v: results[v]['steps_per_s'])
+    head = 'learned16' if 'learned16' in ok else max(ok, key=lambda v: results[v]['steps_per_s'])
     hr = results[head]
 
     log('kernel timing sample')
     asm = assembly_roofline(tf, torch, hr['itf'], hr['x'])
+    vec = vector_rooflines(torch, hr['itf'], hr['x'])
     log('profiled repeat: %s' % head)
     pr, kt = profiled_run(tf, torch, spec, head, maxit)
     pid, (calls, ms, work) = max(kt.items(), key=lambda kv: kv[1][1])
     name, bound, wunit = PROFILED[pid]
-    single = head != 'fp64'
+    single = not head.startswith('fp64')
     if bound == 'mfma':
         peak = FP32_PEAK_TFLOPS if single else FP64_PEAK_TFLOPS
         achieved = work / (ms * 1e-3) / 1e12
@@ -363,13 +482,13 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        key = '%s %s' % (name, 'f64' if head == 'fp64' else ('f32' if head == 'mixed' else 'f16'))
+        key = '%s %s' % (name, 'f64' if not single else ('f32' if head == 'mixed' else 'f16'))
         if key in tj:
             traffic = tj[key].get('bytes_per_launch')
     roof = {'bound': bound, 'achieved': achieved, 'peak': peak, 'unit': unit, 'frac': achieved / peak,
             'traffic': traffic,
-            'kernel': '%s (%s)' % (name, 'FP64' if head == 'fp64' else ('FP32' if head == 'mixed'
-                                                                        else 'FP32, FP16 blocks')),
+            'kernel': '%s (%s)' % (name, 'FP64' if not single else ('FP32' if head == 'mixed'
+                                                                       else 'FP32, FP16 blocks')),
             'launches': calls, 'avg_launch_us': ms / max(calls, 1) * 1e3,
             'per_launch_work': work / max(calls, 1), 'work_unit': wunit,
             'profiled_run_steps_per_s': pr['steps_per_s'],
@@ -388,6 +507,8 @@ def main():
         'unit': 'steps/s',
         'n_gpus': 1,
         'steps': hr['accepted'],
+        'steps_note': ('--steps caps the accepted steps (0, the default, runs the whole continuation to the target); '
+                       'the count is what the continuation needed'),
         'warmup': args.warmup,
         'ms_per_step': 1000.0 / hr['steps_per_s'],
         'higher_is_better': True,
@@ -402,11 +523,14 @@ def main():
         },
         'variants': {v: {k: r[k] for k in ('steps_per_s', 'seconds', 'accepted', 'corrector_iterations',
                                            'factorizations', 'solves', 'gmres_iterations',
-                                           'worst_relative_residual', 'background_factorizations')}
+                                           'worst_relative_residual', 'background_factorizations', 'learned',
+                                           'preconditioner', 'device_memory_GB')}
                      for v, r in results.items()},
+        'learned_training': LEARNED.get('info'),
         'final_state_rel_err_vs_fp64': agree,
         'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},
         'assembly': asm,
+        'spmv_gmres': vec,
         'roofline': roof,
         'cpu_baseline': cpu,
     }

@@ -137,7 +137,7 @@ int tf_normalize_f32(int n, const float* w, double* work, double* nrm_dev, float
 
 /* Native restarted FGMRES (the whole solve in one call; Givens rotations on the host).
  * single != 0: FP32 basis and operands.  Preconditioner kind 0 none, 1 tf_bcr, 2 tf_linelu,
- * 3 cell-block inverse (prec = the FP64 inverse blocks, ncell, dof).  The matrix values
+ * 3 cell-block inverse (prec = the FP64 inverse blocks, ncell, dof), 4 tf_mg, 5 tf_lineilu.  The matrix values
  * have the handle's precision.  Replaces scipy.sparse.linalg.gmres (SciPy.py:343-370). */
 typedef struct tf_krylov tf_krylov;
 int tf_krylov_create(tf_krylov** out, int n, int restart, int single);
@@ -190,6 +190,41 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream);
  * MFMA over three-term BF16 splits of the FP32 operands (hi*hi + hi*lo + lo*hi, FP32
  * accumulation, about 2^-16 relative per product); 1: one term (no reference counterpart). */
 int tf_bcr_set_gemm_split(tf_bcr* h, int split);
+/* on != 0 (before the first factorisation): the factorisation also forms Y L and Y U of the
+ * lines eliminated at levels >= 1 in place of their L and U, and each backward level of the apply
+ * is one kernel (x_e = Y r_e - (Y L) x_a - (Y U) x_b) instead of two (no reference counterpart:
+ * worth it for factors applied many times, such as the lagged FP32 factor of the mixed solver). */
+int tf_bcr_set_fused(tf_bcr* h, int on);
+
+/* ---- block ILU over grid x-lines, four-colour line order (two colours in 2D) ----
+ * M = (P + L) P^-1 (P + U) with dense m x m pivots P (m = nx dof) and the Jacobian's own
+ * sparse couplings between lines; all lines of one colour are factored and applied as one
+ * batch.  single != 0: FP32 factor and FP32 vectors.  Replaces, as a preconditioner, the
+ * incomplete LU of SciPy._compute_preconditioner (scipy spilu, SciPy.py:164-192). */
+typedef struct tf_lineilu tf_lineilu;
+int tf_lineilu_create(tf_lineilu** out, int nx, int ny, int nz, int dof, int single);
+int tf_lineilu_destroy(tf_lineilu* h);
+int tf_lineilu_factor(tf_lineilu* h, const int* row_ptr, const int* col, const double* val, int gauge, void* stream);
+/* flags[0]: couplings beyond the lists (structure error); flags[1]: perturbed pivots.  Synchronises. */
+int tf_lineilu_status(tf_lineilu* h, int* flags_host, void* stream);
+int tf_lineilu_solve_f64(tf_lineilu* h, const double* r, double* x, void* stream);
+int tf_lineilu_solve_f32(tf_lineilu* h, const float* r, float* x, void* stream);
+
+/* ---- two-grid preconditioner (3D cavity grids whose exact factor does not fit) ----
+ * V(1,1) cycle: line block ILU smoothing on the grid, exact solve (tf_bcr) of the Jacobian
+ * re-assembled on the grid coarsened by two in every direction at the restricted state.
+ * Replaces, as the preconditioner of GMRES, scipy's spilu (SciPy.py:164-192,260-309).
+ * tf_mg_restrict_state: the coarse state (scalars: mean of the fine cells; velocities: mean
+ * of the coincident fine faces).  tf_mg_set: the smoother, the fine gauged Jacobian (values
+ * in the handle's precision) and the coarse factor the cycle uses (pointers are kept). */
+typedef struct tf_mg tf_mg;
+int tf_mg_restrict_state(int nx, int ny, int nz, int dim, int dof, const double* fine, double* coarse, void* stream);
+int tf_mg_create(tf_mg** out, int nx, int ny, int nz, int dim, int dof, int single);
+int tf_mg_destroy(tf_mg* h);
+int tf_mg_set(tf_mg* h, tf_lineilu* smoother, const int* row_ptr, const int* col, const void* val, int gauge_fine,
+              tf_bcr* coarse, int gauge_coarse);
+int tf_mg_solve_f64(tf_mg* h, const double* r, double* z, void* stream);
+int tf_mg_solve_f32(tf_mg* h, const float* r, float* z, void* stream);
 
 /* ---- learned preconditioner: convolutions on the cell grid (no reference counterpart) ----
  * Activations cell-major [ncell][C] in BF16 (uint16 storage); layer weights in the

@@ -416,3 +416,52 @@ def test_checkpoint_resume(hip, tmp_path):
     a, b = itf2.array_from_vector(xr), itf3.array_from_vector(xu)
     assert abs(mur - 1000) <= 1e-3 and abs(muu - 1000) <= 1e-3
     assert numpy.linalg.norm(a - b) / numpy.linalg.norm(b) <= 1e-6
+
+
+def test_use_iterative_solver_options(hip):
+    """The reference's iterative-solver settings (tests/test_SciPy.py:32-67):
+    'Use Iterative Solver' with 'Convergence Tolerance' 1e-8 gives a solve to
+    that relative residual whose velocities recover x from b = J x to
+    10 tol ||b|| (the pressure only up to its gauge)."""
+    dim, dof, nx = 3, 4, 4
+    params = {'Use Iterative Solver': True, 'Iterative Solver': {'Convergence Tolerance': 1e-8}}
+    itf = hip.Interface(params, nx, nx, nx, dim, dof)
+    n = itf.n
+    x = numpy.random.default_rng(7).random(n)
+    A = itf.jacobian(numpy.zeros(n))
+    b = A.to_scipy() @ x
+    y = itf.array_from_vector(itf.solve(A, b))
+    its, rel, name, _ = itf.solver.log[-1]
+    tol = 1e-8 * numpy.linalg.norm(b) * 10
+    y[dim::dof] += x[dim]
+    print('iterative settings: %d iterations, rel res %.2e (%s)' % (its, rel, name), flush=True)
+    assert rel <= 1e-8
+    for d in range(dim):
+        assert numpy.linalg.norm(y[d::dof] - x[d::dof]) < tol
+
+
+@pytest.mark.parametrize('single', [False, True])
+@pytest.mark.parametrize('shape', [(48, 48, 1), (24, 17, 1), (6, 6, 6)])
+def test_fused_apply_matches_two_pass(hip, single, shape):
+    """A factor made with Y L, Y U in place of L, U (one backward kernel per level)
+    applies the same inverse as the two-pass factor: equal to rounding, and the
+    solve with it still meets 1e-10."""
+    import torch
+    from transiflow_amd.solver import LineFactor
+    itf = hip.Interface({'Reynolds Number': 500}, *shape)
+    g = itf.solver.gauge()
+    J = itf.jacobian(seeded_state(itf.n, 61) * 0.2)
+    r = torch.from_numpy(seeded_state(itf.n, 62)).cuda()
+    r[g] = 0.0
+    out = []
+    for fused in (False, True):
+        f = LineFactor(itf, J, g, single=single, factor=False)
+        f.set_fused(fused)
+        f.refactor(J, g)
+        rv = r.float() if single else r
+        z = torch.empty_like(rv)
+        (f.apply32 if single else f.apply)(rv, z)
+        out.append(z.double().cpu().numpy())
+    diff = numpy.linalg.norm(out[1] - out[0]) / numpy.linalg.norm(out[0])
+    print('%s %s: fused vs two-pass apply %.2e' % (shape, 'fp32' if single else 'fp64', diff), flush=True)
+    assert diff <= (1e-3 if single else 1e-10)

@@ -93,6 +93,19 @@ SIGNATURES = {
     'tf_bcr_set_graph': (_c_int, [_vp, _c_int]),
     'tf_bcr_set_half': (_c_int, [_vp, _c_int, _vp]),
     'tf_bcr_set_gemm_split': (_c_int, [_vp, _c_int]),
+    'tf_bcr_set_fused': (_c_int, [_vp, _c_int]),
+    'tf_lineilu_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int]),
+    'tf_lineilu_destroy': (_c_int, [_vp]),
+    'tf_lineilu_factor': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_lineilu_status': (_c_int, [_vp, _ip, _vp]),
+    'tf_lineilu_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_lineilu_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_mg_restrict_state': (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    'tf_mg_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    'tf_mg_destroy': (_c_int, [_vp]),
+    'tf_mg_set': (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int]),
+    'tf_mg_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_mg_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
     'tf_fill_f32': (_c_int, [_c_long, _c_double, _vp, _vp]),
     'tf_axpby_f32': (_c_int, [_c_long, _c_double, _vp, _c_double, _vp, _vp, _vp]),
     'tf_scale_f32': (_c_int, [_c_long, _c_double, _vp, _vp, _vp]),

@@ -43,6 +43,8 @@ SOURCES = {
     'gemm.hip': [],
     'conv.hip': [],
     'gmres.hip': [],
+    'lineilu.hip': [],
+    'mg.hip': [],
 }
 
 

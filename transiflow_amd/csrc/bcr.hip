@@ -33,14 +33,20 @@
 // Replaces the sparse LU of the reference's direct solve
 // (scipy.sparse.linalg.splu, transiflow/interface/SciPy.py:131-162,194-258).
 #include "gemm.h"
+#include "gj.h"
 #include "tf_common.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
 namespace {
 
-constexpr int PB = 64;  // Gauss-Jordan panel width
+using tf::gj::PB;
+using tf::gj::k_panel;
+using tf::gj::k_panel_copy;
+using tf::gj::k_panel_store;
+using tf::gj::readlane_t;
 constexpr int KC = 8;   // couplings per row/column to a neighbouring line
 
 struct Cpl {
@@ -161,139 +167,6 @@ __global__ __launch_bounds__(256) void k_diag_blocks(int m, Order o, const int* 
     const long cl = (long)o.to_line(c) - (long)j * m;
     if (cl >= 0 && cl < m) row[cl] = (T)v[k];
   }
-}
-
-// ---- batched Gauss-Jordan inversion of m x m matrices ------------------------
-
-template <typename T>
-__device__ __forceinline__ T readlane_t(T v, int lane);
-template <>
-__device__ __forceinline__ double readlane_t<double>(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-template <>
-__device__ __forceinline__ float readlane_t<float>(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-// Inverse of the pb x pb diagonal panel block of X_b at k0 (padded with the
-// identity to 64): Gauss-Jordan on [A | I], implicit partial pivoting; 8 waves,
-// lane = row, 16 augmented columns per wave in registers.  One block per matrix.
-template <typename T>
-__global__ __launch_bounds__(512) void k_panel(int m, int k0, int pb, T* const* __restrict__ Xp,
-                                               T* __restrict__ Pinv, int* __restrict__ nperturb) {
-  __shared__ T colk[2][64];
-  __shared__ T wmax[8];
-  const T* X = Xp[blockIdx.x];
-  T* P = Pinv + (long)blockIdx.x * PB * PB;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  T a[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = wave * 16 + q;
-    T v;
-    if (c < 64)
-      v = (lane < pb && c < pb) ? X[(long)(k0 + lane) * m + k0 + c] : (lane == c ? (T)1 : (T)0);
-    else
-      v = (lane == c - 64) ? (T)1 : (T)0;
-    a[q] = v;
-  }
-  T mx = 0;
-  if (wave < 4) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) mx = fmax(mx, fabs(a[q]));
-  }
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-  if (lane == 0) wmax[wave] = mx;
-  __syncthreads();
-  T amax = 0;
-  for (int w = 0; w < 4; ++w) amax = fmax(amax, wmax[w]);
-  if (amax == (T)0) amax = 1;
-  const T tol = (sizeof(T) == 8 ? (T)1e-13 : (T)1e-6) * amax;
-  bool used = false;
-  int mystep = -1;
-  for (int k = 0; k < 64; ++k) {
-    if (wave == (k >> 4)) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q == (k & 15)) colk[k & 1][lane] = a[q];
-    }
-    __syncthreads();
-    const T f = colk[k & 1][lane];
-    // pivot: largest |f| among unused rows; 32-bit key (magnitude bits, lane) reduced with shuffles
-    // (unused rows get keys >= 64, ties go to the lower lane; used rows key 0)
-    unsigned key = 0;
-    if (!used) {
-      const unsigned bits = __float_as_uint((float)fabs(f));
-      key = (bits & ~63u) + 64u + (unsigned)(63 - lane);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned o = __shfl_xor(key, off, 64);
-      key = o > key ? o : key;
-    }
-    const int p = __builtin_amdgcn_readfirstlane(63 - (int)((key - 64u) & 63u));
-    T piv = readlane_t<T>(f, p);
-    if (fabs(piv) < tol) {
-      piv = piv < (T)0 ? -tol : tol;
-      if (threadIdx.x == 0) atomicAdd(nperturb, 1);
-    }
-    const T ip = (T)1 / piv;
-    if (lane == p) {
-      used = true;
-      mystep = k;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const T pr = readlane_t<T>(a[q], p) * ip;
-      a[q] = (lane == p) ? pr : a[q] - f * pr;
-    }
-  }
-  if (wave >= 4 && mystep >= 0 && mystep < pb) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = wave * 16 + q - 64;
-      if (c < pb) P[mystep * pb + c] = a[q];
-    }
-  }
-}
-
-// Snapshots for panel K = [k0, k0+pb) of X_b:
-//   Kr = X[K, :] with X[K, K] replaced by the identity   (pb x m)
-//   Kc = X[:, K] with the rows K zeroed                    (m x pb)
-// and X[:, K] is zeroed (the update rebuilds it).
-template <typename T>
-__global__ __launch_bounds__(256) void k_panel_copy(int m, int k0, int pb, T* const* __restrict__ Xp,
-                                                    T* __restrict__ Krb, T* __restrict__ Kcb) {
-  T* X = Xp[blockIdx.y];
-  T* Kr = Krb + (long)blockIdx.y * PB * m;
-  T* Kc = Kcb + (long)blockIdx.y * m * PB;
-  const long total = (long)pb * m;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const int s = (int)(e / m), c = (int)(e % m);
-    const bool inK = c >= k0 && c < k0 + pb;
-    Kr[(long)s * m + c] = inK ? (c - k0 == s ? (T)1 : (T)0) : X[(long)(k0 + s) * m + c];
-  }
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(e / pb), s = (int)(e % pb);
-    const bool rowK = i >= k0 && i < k0 + pb;
-    T* x = X + (long)i * m + k0 + s;
-    Kc[(long)i * pb + s] = rowK ? (T)0 : *x;
-    *x = (T)0;
-  }
-}
-
-// X[K, :] = Tbuf  (after the rank-pb update)
-template <typename T>
-__global__ __launch_bounds__(256) void k_panel_store(int m, int k0, int pb, T* const* __restrict__ Xp,
-                                                     const T* __restrict__ Tb) {
-  T* X = Xp[blockIdx.y];
-  const T* Tm = Tb + (long)blockIdx.y * PB * m;
-  const long total = (long)pb * m;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
-    X[(long)(k0 + e / m) * m + e % m] = Tm[e];
 }
 
 // ---- level-0 sparse products -----------------------------------------------
@@ -542,6 +415,34 @@ __global__ __launch_bounds__(256) void k_bwd_x(int m, int G, int vec, const int*
   if (live && g == 0) x[base + r] = (V)acc;
 }
 
+// Levels >= 1 of a factor made with the products Y L and Y U in place of L and U
+// (tf_bcr_set_fused): x[e] = Y_e rw[e] - (Y_e L_e) x[a] - (Y_e U_e) x[b] in one
+// pass, the same three blocks read and no intermediate vector.
+template <typename B, typename V>
+__global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const int* __restrict__ e_line,
+                                                   const int* __restrict__ a_line, const int* __restrict__ b_line,
+                                                   B Yb, B YLb, B YUb, const V* __restrict__ rw, V* __restrict__ x) {
+  const int bi = blockIdx.y;
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
+  const bool live = r < m;
+  const int rr = live ? r : 0;
+  const long base = (long)e_line[bi] * m;
+  const int a = a_line[bi], bb = b_line[bi];
+  double acc = row_dot(Yb, bi, rr, rw + base, m, g, G, vec);
+  if (a >= 0) acc -= row_dot(YLb, bi, rr, x + (long)a * m, m, g, G, vec);
+  if (bb >= 0) acc -= row_dot(YUb, bi, rr, x + (long)bb * m, m, g, G, vec);
+  if (live && g == 0) x[base + r] = (V)acc;
+}
+
+// copy a batch of m x m blocks: dst[b] = src[b]   (grid.y = batch)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_blocks(long mm, T* const* __restrict__ src, T* const* __restrict__ dst) {
+  const T* s = src[blockIdx.y];
+  T* d = dst[blockIdx.y];
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < mm; e += (long)gridDim.x * blockDim.x) d[e] = s[e];
+}
+
 // Level 0, kept lines: rw[k] -= L_k y[e1] + U_k y[e2] with the sparse couplings and
 // y[e] = Y_e rw[e] (so P1 rw[e1] = L_k Y_e1 rw[e1] without reading P1).  Thread per row.
 template <typename V>
@@ -661,6 +562,10 @@ struct BcrLevelDev {
   void **g_fa_A, **g_fa_B, **g_fa_C;  // L_k = -P1 L_e1
   void **g_fb_A, **g_fb_B, **g_fb_C;  // U_k = -P2 U_e2
   void **z_fa, **z_fb;                // L_k (U_k) that become zero
+  // fused factors: L_e <- Y_e L_e, U_e <- Y_e U_e (through the scratch W) at levels >= 1
+  int n_yl, n_yu;
+  void **g_yl_A, **g_yl_B, **g_yl_C;  // Y_e, L_e, W
+  void **g_yu_A, **g_yu_B, **g_yu_C;  // Y_e, U_e, W
   // FP16 copies read by the FP32 apply (tf_bcr_set_half): blocks and row scales
   void **Yh, **Lh, **Uh, **P1h, **P2h;
   void **Ys, **Ls, **Us, **P1s, **P2s;
@@ -713,6 +618,8 @@ struct tf_bcr {
   size_t slots;
   size_t solve_bytes_half;
   int split;           // level GEMMs of an FP32 factor on BF16 terms: 0 off, 1 or 3 terms
+  int fused;           // L, U of the levels >= 1 hold Y L, Y U (one backward kernel per level)
+  void* W;             // scratch of the fused products (largest level, m^2 each)
 };
 
 namespace {
@@ -828,6 +735,19 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
       level_gemm<T>(h, m, m, m, L.g_p2_C, m, L.g_d2_B, m, L.g_d2_A, m, -1.0, 1.0, L.n_p2, st);
       level_gemm<T>(h, m, m, m, L.g_fa_A, m, L.g_fa_B, m, L.g_fa_C, m, -1.0, 0.0, L.n_fa, st);
       level_gemm<T>(h, m, m, m, L.g_fb_A, m, L.g_fb_B, m, L.g_fb_C, m, -1.0, 0.0, L.n_fb, st);
+      if (h->fused) {
+        // the eliminated lines' L_e, U_e are not used by the factorisation after
+        // this level: replace them by Y_e L_e and Y_e U_e for the one-pass backward
+        const long mm2 = (long)m * m;
+        gemm<T>(m, m, m, L.g_yl_A, m, L.g_yl_B, m, L.g_yl_C, m, 1.0, 0.0, L.n_yl, st);
+        if (L.n_yl)
+          hipLaunchKernelGGL(k_copy_blocks<T>, dim3(256, L.n_yl), dim3(256), 0, st, mm2, (T* const*)L.g_yl_C,
+                             (T* const*)L.g_yl_B);
+        gemm<T>(m, m, m, L.g_yu_A, m, L.g_yu_B, m, L.g_yu_C, m, 1.0, 0.0, L.n_yu, st);
+        if (L.n_yu)
+          hipLaunchKernelGGL(k_copy_blocks<T>, dim3(256, L.n_yu), dim3(256), 0, st, mm2, (T* const*)L.g_yu_C,
+                             (T* const*)L.g_yu_B);
+      }
     }
     // couplings that no longer exist
     const long mm = (long)m * m;
@@ -873,6 +793,11 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
+    if (l > 0 && h->fused) {
+      hipLaunchKernelGGL((k_bwd_fused<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Yb(L),
+                         Lb(L), Ub(L), (const V*)rw, xo);
+      continue;
+    }
     if (l == 0)
       hipLaunchKernelGGL((k_bwd_t_sparse<V>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr, h->Ur,
                          h->nUr, (const V*)rw, (const V*)xo, tb);
@@ -1069,10 +994,22 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   PoolBuilder pb;
   struct Off {
     size_t e, ea, eb, Ye, Le, Ue, k, k1, k2, P1, P2, p1A, p1B, p1C, d1A, d1B, p2A, p2B, p2C, d2A, d2B, faA, faB,
-        faC, fbA, fbB, fbC, zfa, zfb, hase1, hase2, faline, fbline;
-    int ne, nk, np1, np2, nfa, nfb, nnofa, nnofb;
+        faC, fbA, fbB, fbC, zfa, zfb, hase1, hase2, faline, fbline, ylA, ylB, ylC, yuA, yuB, yuC;
+    int ne, nk, np1, np2, nfa, nfb, nnofa, nnofb, nyl, nyu;
   };
   std::vector<Off> offs;
+  // scratch of the fused products: the largest batch of one level (levels >= 1)
+  size_t wslots = 0;
+  for (size_t l = 1; l < h->plan.size(); ++l) {
+    size_t na = 0, nb = 0;
+    for (size_t b = 0; b < h->plan[l].e.size(); ++b) {
+      na += h->plan[l].ea[b] >= 0;
+      nb += h->plan[l].eb[b] >= 0;
+    }
+    wslots = std::max(wslots, std::max(na, nb));
+  }
+  if (wslots) TF_HIP(hipMalloc(&h->W, bm * wslots));
+  int l_index = 0;
   for (const LevelPlan& L : h->plan) {
     Off o{};
     o.ne = (int)L.e.size();
@@ -1169,7 +1106,33 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
     o.zfa = pb.add_ptrs(zfa);
     o.nnofb = (int)zfb.size();
     o.zfb = pb.add_ptrs(zfb);
+    // fused products Y_e L_e, Y_e U_e of the eliminated lines at levels >= 1 (W: scratch slots)
+    std::vector<void*> ylA, ylB, ylC, yuA, yuB, yuC;
+    if (l_index > 0) {
+      for (int b = 0; b < o.ne; ++b) {
+        const int e = L.e[b];
+        if (L.ea[b] >= 0) {
+          ylC.push_back(blk((char*)h->W, (int)ylA.size()));
+          ylA.push_back(blk(D, e));
+          ylB.push_back(blk(Lm, e));
+        }
+        if (L.eb[b] >= 0) {
+          yuC.push_back(blk((char*)h->W, (int)yuA.size()));
+          yuA.push_back(blk(D, e));
+          yuB.push_back(blk(Um, e));
+        }
+      }
+    }
+    o.nyl = (int)ylA.size();
+    o.nyu = (int)yuA.size();
+    o.ylA = pb.add_ptrs(ylA);
+    o.ylB = pb.add_ptrs(ylB);
+    o.ylC = pb.add_ptrs(ylC);
+    o.yuA = pb.add_ptrs(yuA);
+    o.yuB = pb.add_ptrs(yuB);
+    o.yuC = pb.add_ptrs(yuC);
     offs.push_back(o);
+    ++l_index;
   }
   // one solve reads every Y once, the dense L/U of the lines eliminated at
   // levels >= 1, the multipliers of the kept lines at levels >= 1, the level-0
@@ -1237,6 +1200,14 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
     L.z_fa = (void**)(P + o.zfa);
     L.n_nofb = o.nnofb;
     L.z_fb = (void**)(P + o.zfb);
+    L.n_yl = o.nyl;
+    L.g_yl_A = (void**)(P + o.ylA);
+    L.g_yl_B = (void**)(P + o.ylB);
+    L.g_yl_C = (void**)(P + o.ylC);
+    L.n_yu = o.nyu;
+    L.g_yu_A = (void**)(P + o.yuA);
+    L.g_yu_B = (void**)(P + o.yuB);
+    L.g_yu_C = (void**)(P + o.yuC);
     h->lev.push_back(L);
   }
   return 0;
@@ -1253,7 +1224,7 @@ int tf_bcr_destroy(tf_bcr* h) {
   (void)hipFree(h->gout);
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
-                  (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool})
+                  (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;
@@ -1425,6 +1396,18 @@ int tf_bcr_set_gemm_split(tf_bcr* h, int split) {
   if (split != 0 && split != 1 && split != 3) return tf::fail("split must be 0, 1 or 3");
   if (split && !h->single) return tf::fail("BF16 GEMM terms need a single-precision factor");
   h->split = split;
+  return 0;
+}
+
+// on != 0: the factorisation stores Y L and Y U in place of the levels' L and U
+// and the backward sweep is one kernel per level (two more GEMMs per eliminated
+// line in the factorisation, one launch and one vector pass less per level in
+// every apply).  Only before the first factorisation.
+int tf_bcr_set_fused(tf_bcr* h, int on) {
+  if (!h) return tf::fail("null handle");
+  if (h->factored && (on != 0) != (h->fused != 0))
+    return tf::fail("tf_bcr_set_fused: the handle already holds a factor of the other form");
+  h->fused = on ? 1 : 0;
   return 0;
 }
 

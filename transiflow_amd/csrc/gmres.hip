@@ -53,6 +53,12 @@ int precond(const tf_krylov* k, int kind, void* handle, int ncell, int dof, cons
     case 3:
       return k->single ? tf_cellblock_apply_f32(ncell, dof, (const double*)handle, (const float*)r, (float*)z, st)
                        : tf_cellblock_apply_f64(ncell, dof, (const double*)handle, (const double*)r, (double*)z, st);
+    case 4:
+      return k->single ? tf_mg_solve_f32((tf_mg*)handle, (const float*)r, (float*)z, st)
+                       : tf_mg_solve_f64((tf_mg*)handle, (const double*)r, (double*)z, st);
+    case 5:
+      return k->single ? tf_lineilu_solve_f32((tf_lineilu*)handle, (const float*)r, (float*)z, st)
+                       : tf_lineilu_solve_f64((tf_lineilu*)handle, (const double*)r, (double*)z, st);
     default:
       return hipMemcpyAsync(z, r, k->es * (size_t)k->n, hipMemcpyDeviceToDevice, st) == hipSuccess
                  ? 0

@@ -146,7 +146,12 @@ class Interface:
         col = T.empty(max(nnz.value, 1), dtype=T.int32, device='cuda')[:nnz.value]
         val = T.empty(max(nnz.value, 1), dtype=T.float64, device='cuda')[:nnz.value]
         _lib.call('tf_jacobian_fill', self._handle, ptr(x.data), ptr(row_ptr), ptr(col), ptr(val), stream())
-        return DeviceCsrMatrix(row_ptr, col, val, self.n)
+        J = DeviceCsrMatrix(row_ptr, col, val, self.n)
+        # where the Jacobian was taken: the two-grid preconditioner re-assembles it
+        # on the coarse grid at the restricted state with the same parameters
+        J.state = x.data.clone()
+        J.parameters = dict(self.parameters)
+        return J
 
     def mass_matrix(self):
         T = torch()

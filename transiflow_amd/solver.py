@@ -9,11 +9,18 @@ reference's splu solve is matched to that relative residual).
 
 Options (``parameters['HIP Solver']``, all optional):
 
-``'Preconditioner'``   ``'line'`` (exact block-tridiagonal factorisation over
-                       grid lines by block cyclic reduction, default when the
-                       dense line blocks fit the memory budget), ``'thomas'``
-                       (the same factorisation by the sequential block Thomas
-                       recursion), ``'block-jacobi'`` (cell blocks);
+``'Preconditioner'``   ``'auto'`` (default: ``'line'`` when its dense blocks fit
+                       the memory budget, else ``'multigrid'`` for 3D grids,
+                       else a SolverError naming the sizes), ``'line'`` (exact
+                       block-tridiagonal factorisation over grid lines -- 2D --
+                       or planes -- 3D -- by block cyclic reduction),
+                       ``'thomas'`` (the same factorisation by the sequential
+                       block Thomas recursion), ``'multigrid'`` (two grids:
+                       line block ILU smoothing, exact factor of the Jacobian
+                       re-assembled on the grid coarsened by two),
+                       ``'line-ilu'`` (the four-colour line block ILU alone),
+                       ``'block-jacobi'`` (cell blocks; does not converge on
+                       these saddle-point systems beyond small cases);
 ``'Precision'``        ``'fp64'`` (default) or ``'mixed'``: FP64 iterative
                        refinement (residual b - A x, update and convergence test in
                        FP64) around an FP32 inner FGMRES (FP32 basis, SpMV with FP32
@@ -49,6 +56,15 @@ Options (``parameters['HIP Solver']``, all optional):
                        ``'bf16'``: the level GEMMs of the FP32 factor on FP32
                        MFMA, or on BF16 MFMA over three-term (one-term) BF16
                        splits of the FP32 operands;
+``'Reuse Factor'``     (fp64) False (default) or True: reuse the factor of an
+                       earlier Jacobian as the mixed precision does, refactoring
+                       when a solve takes more than 'Reuse Limit' iterations
+                       scaled from 'Inner Tolerance' to 'Tolerance' (24 by
+                       default) or does not converge;
+``'Fused Apply'``      (line factor) True or False (default: True for a factor
+                       that is reused -- mixed, or 'Reuse Factor' -- else
+                       False): the factorisation also forms Y L and Y U so
+                       that each backward level of the apply is one kernel;
 ``'Background Factor'`` (mixed, line factor) False (default) or True: when the
                        lagged factor exceeds the reuse limit, a second handle
                        factors the Jacobian on a side stream while the inner
@@ -57,6 +73,14 @@ Options (``parameters['HIP Solver']``, all optional):
                        factorisation overlaps the HBM-bound applies; iteration
                        counts then depend on timing, results still meet
                        ``'Tolerance'``).
+
+With the reference's ``'Use Iterative Solver': True`` the defaults become
+those of its iterative path (SciPy.py:260-309): ``'Iterative Solver'``'s
+``'Convergence Tolerance'`` (1e-6), ``'Restart'`` (100) and ``'Maximum
+Iterations'`` (1000), and a solve that misses the tolerance returns its last
+iterate instead of raising (``'Fail On Tolerance'`` False); the reference's
+``'Preconditioner'`` settings belong to scipy's spilu and have no counterpart
+here (the preconditioner is chosen as above).
 
 Every solve appends (iterations, relative residual, preconditioner name,
 factorised?) to ``Solver.log`` for measurement.
@@ -82,12 +106,19 @@ class SolverError(RuntimeError):
 
 def _opts(parameters):
     o = dict(parameters.get('HIP Solver', {}))
+    # the reference's iterative-solver settings (SciPy.py:260-309: restarted GMRES
+    # with restart 100, at most 1000 iterations, relative tolerance 1e-6 from
+    # 'Iterative Solver', failures ignored) when 'Use Iterative Solver' is set;
+    # 'HIP Solver' entries still take precedence
+    it = parameters.get('Use Iterative Solver', False)
+    ip = dict(parameters.get('Iterative Solver', {})) if it else {}
     return {
         'Preconditioner': o.get('Preconditioner', 'auto'),
         'Precision': o.get('Precision', 'fp64'),
-        'Tolerance': float(o.get('Tolerance', 1e-10)),
-        'Restart': int(o.get('Restart', 30)),
-        'Maximum Iterations': int(o.get('Maximum Iterations', 300)),
+        'Tolerance': float(o.get('Tolerance', ip.get('Convergence Tolerance', 1e-6) if it else 1e-10)),
+        'Restart': int(o.get('Restart', ip.get('Restart', 100) if it else 30)),
+        'Maximum Iterations': int(o.get('Maximum Iterations', ip.get('Maximum Iterations', 1000) if it else 300)),
+        'Fail On Tolerance': bool(o.get('Fail On Tolerance', not it)),
         'Reuse Limit': int(o.get('Reuse Limit', 8)),
         'Inner Tolerance': float(o.get('Inner Tolerance', 5e-4)),
         'Inner Iterations': int(o.get('Inner Iterations', 10)),
@@ -97,10 +128,20 @@ def _opts(parameters):
         'Apply Storage': o.get('Apply Storage', 'fp16'),
         'Background Factor': bool(o.get('Background Factor', False)),
         'Factor GEMM': o.get('Factor GEMM', 'fp32'),
+        'Reuse Factor': bool(o.get('Reuse Factor', False)),
+        'Fused Apply': o.get('Fused Apply'),
     }
 
 
 _SPLITS = {'fp32': 0, 'bf16x3': 3, 'bf16': 1}
+
+
+def _fused_apply(opts, single):
+    """One-kernel backward levels (two more GEMMs per line at factor time) pay for a
+    factor applied many times: the lagged factor of the mixed or factor-reusing
+    solves; a factor made for every Jacobian is applied only a few times."""
+    o = opts['Fused Apply']
+    return (single or opts['Reuse Factor']) if o is None else bool(o)
 
 
 def _configure_single(f, opts):
@@ -199,6 +240,13 @@ class LineFactor:
             _lib.call('tf_bcr_set_half', self._h, 1 if on else 0, stream())
             self.half = bool(on)
 
+    def set_fused(self, on):
+        """Form Y L and Y U at factor time so that each backward level of the apply
+        is one kernel (block cyclic reduction; before the first factorisation)."""
+        if self._p == 'tf_bcr':
+            _lib.call('tf_bcr_set_fused', self._h, 1 if on else 0)
+            self.fused = bool(on)
+
     def set_gemm_split(self, split):
         """Level GEMMs of the FP32 factor on BF16 terms (0 off, 1 or 3 terms); applies
         from the next factorisation on."""
@@ -225,6 +273,154 @@ class LineFactor:
             self._h = None
 
 
+class LineILU:
+    """Block ILU over grid x-lines with a four-colour line order (csrc/lineilu.hip):
+    dense inverted line pivots, the Jacobian's own sparse couplings between lines."""
+    name = 'line-ilu'
+
+    def __init__(self, itf, single):
+        h = ctypes.c_void_p()
+        _lib.call('tf_lineilu_create', ctypes.byref(h), itf.nx, itf.ny, itf.nz, itf.dof, 1 if single else 0)
+        self._h = h
+        self.single = single
+        self.owner = None
+        self.perturbed = 0
+
+    def refactor(self, jac, gauge):
+        _take_ownership(self, jac)
+        _lib.call('tf_lineilu_factor', self._h, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(gauge), stream())
+        flags = (ctypes.c_int * 2)()
+        _lib.call('tf_lineilu_status', self._h, flags, stream())
+        if flags[0]:
+            raise SolverError('line ILU: more couplings between lines than the factor keeps (flags %d)' % flags[0])
+        self.perturbed = flags[1]
+
+    def apply(self, r, out):
+        _lib.call('tf_lineilu_solve_f64', self._h, ptr(r), ptr(out), stream())
+        return out
+
+    def apply32(self, r, out):
+        _lib.call('tf_lineilu_solve_f32', self._h, ptr(r), ptr(out), stream())
+        return out
+
+    def native(self):
+        return 5, self._h, 0, 0
+
+    def __del__(self):
+        _destroy(self, 'tf_lineilu_destroy')
+
+
+def coarse_coordinates(x, n):
+    """Coordinate vector (length n/2 + 3, the layout of utils.py:175-232) of the
+    grid coarsened by two: coarse face I is fine face 2I+1; the ghost entries
+    mirror the first and last coarse cells."""
+    N = n // 2
+    X = numpy.empty(N + 3)
+    X[:N] = x[1:n:2]
+    X[-1] = x[-1]
+    X[-2] = X[-1] - (X[0] - X[-1])
+    X[N] = X[N - 1] + (X[N - 1] - X[N - 2]) if N > 1 else X[N - 1] + (X[N - 1] - X[-1])
+    return X
+
+
+def multigrid_fits(itf, single=False):
+    """The two-grid method needs a 3D grid with even sizes and non-periodic
+    walls, and the exact factor of the coarse grid within the memory budget."""
+    d = itf.discretization
+    if itf.nz == 1 or itf.dim != 3 or any(n % 2 for n in (itf.nx, itf.ny, itf.nz)):
+        return False
+    if d.x_periodic or d.y_periodic or d.z_periodic:
+        return False
+    m = (itf.nx // 2) * (itf.ny // 2) * itf.dof
+    return 5 * (itf.nz // 2) * m * m * (4 if single else 8) <= LINE_BUDGET_BYTES
+
+
+class Multigrid:
+    """Two-grid preconditioner (csrc/mg.hip): line block ILU smoothing on the grid
+    and the exact plane factorisation of the Jacobian re-assembled on the grid
+    coarsened by two at the restricted state, one V(1,1) cycle per application.
+    For 3D grids whose own plane factor does not fit (the 64^3 cavity: 16384-row
+    planes); the line ILU alone converges ever more slowly as the grid grows."""
+    name = 'multigrid'
+
+    def __init__(self, itf, single):
+        from .interface import Interface
+        T = torch()
+        self.itf = itf
+        self.single = single
+        self.owner = None
+        nc = (itf.nx // 2, itf.ny // 2, itf.nz // 2)
+        d = itf.discretization
+        self.coarse = Interface(dict(itf.parameters), *nc, itf.dim, itf.dof,
+                                x=coarse_coordinates(itf.x, itf.nx), y=coarse_coordinates(itf.y, itf.ny),
+                                z=coarse_coordinates(itf.z, itf.nz), boundary_conditions=d.boundary_conditions)
+        self.xc = T.empty(self.coarse.n, dtype=T.float64, device='cuda')
+        self.smoother = LineILU(itf, single)
+        self.cf = None
+        h = ctypes.c_void_p()
+        _lib.call('tf_mg_create', ctypes.byref(h), itf.nx, itf.ny, itf.nz, itf.dim, itf.dof, 1 if single else 0)
+        self._h = h
+        self._keep = None
+
+    def refactor(self, jac, gauge):
+        itf, c = self.itf, self.coarse
+        _take_ownership(self, jac)
+        _lib.call('tf_mg_restrict_state', itf.nx, itf.ny, itf.nz, itf.dim, itf.dof, ptr(jac.state), ptr(self.xc),
+                  stream())
+        c.parameters.clear()
+        c.parameters.update(jac.parameters)
+        Jc = c.jacobian(DeviceVector(self.xc))
+        gc = c.pressure_row if (c.dof > c.dim and c.pressure_row is not None) else -1
+        if self.cf is None:
+            self.cf = LineFactor(c, Jc, gc, single=self.single, kind='line', factor=False)
+            opts = _opts(itf.parameters)
+            self.cf.set_fused(_fused_apply(opts, self.single))
+            self.cf.set_graph(opts['Apply Graph'])
+            if self.single:
+                _configure_single(self.cf, opts)
+        self.cf.refactor(Jc, gc)
+        self.smoother.refactor(jac, gauge)
+        val = jac.values32() if self.single else jac.val
+        # the cycle reads these arrays: keep them alive while this factor is in use
+        self._keep = (jac.row_ptr, jac.col, val, Jc)
+        _lib.call('tf_mg_set', self._h, self.smoother._h, ptr(jac.row_ptr), ptr(jac.col), ptr(val), int(gauge),
+                  self.cf._h, int(gc))
+        self.perturbed = self.smoother.perturbed + getattr(self.cf, 'perturbed', 0)
+
+    def apply(self, r, out):
+        _lib.call('tf_mg_solve_f64', self._h, ptr(r), ptr(out), stream())
+        return out
+
+    def apply32(self, r, out):
+        _lib.call('tf_mg_solve_f32', self._h, ptr(r), ptr(out), stream())
+        return out
+
+    def native(self):
+        return 4, self._h, 0, 0
+
+    def __del__(self):
+        _destroy(self, 'tf_mg_destroy')
+
+
+def _take_ownership(f, jac):
+    """A refactored handle belongs to ``jac``; the Jacobian that held it drops it."""
+    old = f.owner() if f.owner is not None else None
+    if old is not None and old is not jac and old.prec is f:
+        old.prec = None
+        old.prec_exact = False
+    f.owner = weakref.ref(jac)
+
+
+def _destroy(obj, fn):
+    h = getattr(obj, '_h', None)
+    if h is not None and h.value:
+        try:
+            getattr(_lib.load(), fn)(h)
+        except Exception:
+            pass
+        obj._h = None
+
+
 def line_block_size(itf):
     """Rows of one grid line (2D x-line) or grid plane (3D)."""
     if itf.nz == 1:
@@ -246,6 +442,20 @@ def line_fits(itf, single=False):
     m = line_layout(itf)[0]
     nlines = itf.n // m
     return 5 * nlines * m * m * (4 if single else 8) <= LINE_BUDGET_BYTES
+
+
+def auto_preconditioner(itf, single):
+    """'line' when the exact line (plane) factor fits the memory budget, else
+    'multigrid' for 3D grids it covers; otherwise no preconditioner that
+    converges is available and the solve fails with a clear error."""
+    if line_fits(itf, single):
+        return 'line'
+    if multigrid_fits(itf, single):
+        return 'multigrid'
+    m, _ = line_layout(itf)
+    raise SolverError('no preconditioner for this grid: the exact line factor needs %.1f GB (budget %.1f GB) and '
+                      'the two-grid method needs a 3D grid with even sizes and walls on every side'
+                      % (5 * itf.n * m * (4 if single else 8) / 1e9, LINE_BUDGET_BYTES / 1e9))
 
 
 class Krylov:
@@ -431,6 +641,7 @@ class BackgroundFactor:
 
     def _configure(self, f):
         opts = _opts(self.solver.itf.parameters)
+        f.set_fused(_fused_apply(opts, True))
         f.set_graph(opts['Apply Graph'])
         _configure_single(f, opts)
         return f
@@ -549,17 +760,25 @@ class Solver:
     def _build_prec(self, jac, kind, single):
         g = self.gauge()
         if kind == 'auto':
-            kind = 'line' if line_fits(self.itf, single) else 'block-jacobi'
+            kind = auto_preconditioner(self.itf, single)
+        if kind in ('multigrid', 'line-ilu'):
+            f = self._factors.get((kind, single))
+            if f is None:
+                f = self._factors[(kind, single)] = (Multigrid if kind == 'multigrid' else LineILU)(self.itf, single)
+            f.refactor(jac, g)
+            return f
         if kind in ('line', 'thomas'):
             # one factor handle per (kind, precision), refactored in place: the
             # dense blocks are GBs, so a new Jacobian reuses the buffers instead
             # of a fresh allocation (the factor of the previous Jacobian goes)
             f = self._factors.get((kind, single))
             if f is None:
+                opts = _opts(self.itf.parameters)
                 f = self._factors[(kind, single)] = LineFactor(self.itf, jac, g, single=single, kind=kind,
                                                                factor=False)
+                f.set_fused(_fused_apply(opts, single))
                 if single:
-                    _configure_single(f, _opts(self.itf.parameters))
+                    _configure_single(f, opts)
             f.refactor(jac, g)
             return f
         if kind == 'block-jacobi':
@@ -573,7 +792,7 @@ class Solver:
         if self._bg is None:
             kind = opts['Preconditioner']
             if kind == 'auto':
-                kind = 'line' if line_fits(self.itf, True) else 'block-jacobi'
+                kind = auto_preconditioner(self.itf, True)
             if kind not in ('line', 'thomas'):
                 return None
             self._bg = BackgroundFactor(self, kind)
@@ -581,6 +800,7 @@ class Solver:
 
     def preconditioner(self, jac, opts):
         mixed = opts['Precision'] == 'mixed'
+        lag = mixed or opts['Reuse Factor']
         bg = self._background(opts)
         if bg is not None:
             bg.poll(jac)
@@ -591,7 +811,7 @@ class Solver:
             return jac.prec, False
         if jac.prec is not None:
             return jac.prec, False
-        if mixed and self._lagged is not None:
+        if lag and self._lagged is not None:
             jac.prec = self._lagged
             jac.prec_exact = False   # factor of an earlier Jacobian
             return jac.prec, False
@@ -599,7 +819,7 @@ class Solver:
         jac.prec = self._build(jac, opts['Preconditioner'], single=mixed)
         jac.prec_exact = True
         jac.prec_time = time.perf_counter() - t0
-        if mixed:
+        if lag:
             self._lagged = jac.prec
         return jac.prec, True
 
@@ -645,7 +865,7 @@ class Solver:
         else:
             its, rel, prec, fresh = self._solve_fp64(jac, b, x, opts)
         self.log.append((its, rel, getattr(prec, 'name', '?'), fresh))
-        if not rel <= opts['Tolerance']:
+        if not rel <= opts['Tolerance'] and opts['Fail On Tolerance']:
             raise SolverError('GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
                               % (rel, its))
         return x
@@ -653,16 +873,31 @@ class Solver:
     def _solve_fp64(self, jac, b, x, opts):
         g = self.gauge()
         prec, fresh = self.preconditioner(jac, opts)
-        if hasattr(prec, 'native'):
-            kry = self._krylov_for(opts['Restart'], native=True)
-            its, rel = kry.solve(jac, jac.val, g, prec, b, x, opts['Tolerance'], opts['Maximum Iterations'])
-            return its, rel, prec, fresh
-        kry = self._krylov_for(opts['Restart'])
 
-        def matvec(v, out):
-            jac.matvec(v, out=out, gauge=g)
+        def run(prec, first):
+            if hasattr(prec, 'native'):
+                kry = self._krylov_for(opts['Restart'], native=True)
+                return kry.solve(jac, jac.val, g, prec, b, x, opts['Tolerance'], opts['Maximum Iterations'],
+                                 x_is_zero=first)
+            kry = self._krylov_for(opts['Restart'])
 
-        its, rel = kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+            def matvec(v, out):
+                jac.matvec(v, out=out, gauge=g)
+
+            return kry.solve(matvec, prec.apply, b, x, opts['Tolerance'], opts['Maximum Iterations'])
+
+        its, rel = run(prec, True)
+        if opts['Reuse Factor'] and not jac.prec_exact:
+            # a lagged factor: refactor once it costs more iterations than the reuse
+            # limit (scaled from 'Inner Tolerance' to 'Tolerance'), and finish the
+            # solve with the new factor when the lagged one did not reach the tolerance
+            limit = opts['Reuse Limit'] * math.log(opts['Tolerance']) / math.log(opts['Inner Tolerance'])
+            if its > limit or not rel <= opts['Tolerance']:
+                self.limit_refactors += 1
+                prec, fresh = self._refactor_now(jac, opts)
+                if not rel <= opts['Tolerance']:
+                    its2, rel = run(prec, False)
+                    its += its2
         return its, rel, prec, fresh
 
     def _learned_for(self, net):
@@ -732,7 +967,9 @@ class Solver:
             p = prec
             net = opts['Learned Network']
             if net is not None:
-                p = self._learned_for(net).bind(p, matvec32)
+                lc = self._learned_for(net)
+                if not lc.off:   # once switched off, the base runs alone (natively)
+                    p = lc.bind(p, matvec32)
             if self.observer is not None:
                 obs, base = self.observer, p
 
