@@ -103,6 +103,12 @@ int tf_set_entry_f64(double* x, long n, long i, double v, void* stream);
  * A's entries keep their order; B's columns missing in A's row are appended. */
 int tf_csr_add_count(int n, const int* ra, const int* ca, const int* rb, const int* cb, int* cnt, int* rc,
                      int* nnz_host, void* stream);
+/* tf_csr_add_count without the cnt argument: the per-row counts go to a scratch array
+ * the library keeps per device (16 MiB, n <= 4,194,304; larger n is an error) and the call
+ * allocates nothing.  No counterpart in the reference: for callers that cannot allocate
+ * (graph capture, allocation-free serving loops).  Calls that overlap on one device race. */
+int tf_csr_add_count_noalloc(int n, const int* ra, const int* ca, const int* rb, const int* cb, int* rc,
+                             int* nnz_host, void* stream);
 int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, double alpha, const int* rb,
                     const int* cb, const double* vb, double beta, const int* rc, int* cc, double* vc, void* stream);
 /* Theta-method step residual, divided by theta dt (TimeIntegration._newton,

@@ -81,6 +81,7 @@ SIGNATURES = {
     'tf_fgmres': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_double, _c_int,
                            _c_int, _ip, _dp, _vp]),
     'tf_csr_add_count': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _vp]),
+    'tf_csr_add_count_noalloc': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _ip, _vp]),
     'tf_csr_add_fill': (_c_int, [_c_int, _vp, _vp, _vp, _c_double, _vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp]),
     'tf_gemm_tuning': (_c_int, [_c_int]),
     'tf_gemm_table': (_c_int, [_ip, _c_int]),

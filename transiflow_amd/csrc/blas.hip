@@ -328,6 +328,47 @@ int tf_csr_add_count(int n, const int* ra, const int* ca, const int* rb, const i
   return 0;
 }
 
+// ---- tf_csr_add_count_noalloc: the count pass with library-owned scratch ----
+// The per-row counts live in a file-scope device array, one copy per device
+// for the life of the process, so the call allocates nothing.  Every call on a
+// device shares that device's array: calls that overlap on one device race.
+// Each call writes the counts of rows 0..n-1 before reading them, so nothing
+// has to be cleared between calls.
+constexpr int CSR_ADD_SCRATCH_ROWS = 1 << 22;  // 4,194,304 rows, 16 MiB per device
+static __device__ __align__(256) int scratch[CSR_ADD_SCRATCH_ROWS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_dev[MAX_DEVICES] = {};  // device address of `scratch`, per device ordinal
+
+// Device address of the current device's `scratch`, looked up once per device
+// (the array's host-side name is not a device address).
+int scratch_of_current_device(int** out) {
+  int dev = 0;
+  TF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return tf::fail("tf_csr_add_count_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    TF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return tf::fail("tf_csr_add_count_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+}  // namespace
+
+int tf_csr_add_count_noalloc(int n, const int* ra, const int* ca, const int* rb, const int* cb, int* rc,
+                             int* nnz_host, void* stream) {
+  if (n < 1 || n > CSR_ADD_SCRATCH_ROWS)
+    return tf::fail("tf_csr_add_count_noalloc: n must be in [1, 4194304] (the size of its scratch)");
+  int* cnt = nullptr;
+  if (int e = scratch_of_current_device(&cnt)) return e;
+  return tf_csr_add_count(n, ra, ca, rb, cb, cnt, rc, nnz_host, stream);
+}
+
 int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, double alpha, const int* rb,
                     const int* cb, const double* vb, double beta, const int* rc, int* cc, double* vc, void* stream) {
   hipLaunchKernelGGL(k_csr_add_fill, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ra, ca, va, alpha,
