@@ -117,6 +117,16 @@ int tf_csr_add_fill(int n, const int* ra, const int* ca, const double* va, doubl
  * and mdiag the diagonal of the mass matrix. */
 int tf_theta_residual(long n, const double* Fv, const double* Fu, const double* mdiag, const double* u,
                       const double* v, double cb, double cm, double* out, void* stream);
+/* Real interleaved form of the complex matrix A + iB (CSR, one row count n): unknown c becomes
+ * (2c, 2c+1) = (re, im); row 2r holds (a, -b), row 2r+1 (b, a) over the union of the row's
+ * columns; row and column ``gauge`` become -e in both halves (the pressure gauge of
+ * SciPy.py:95-103 applied to a complex shifted matrix beta J - alpha M, as the reference's
+ * complex splu solve of JaDa's shifted systems sees it, JaDa.py:146-151).  count: cnt (2n ints
+ * of scratch), rc[0..2n] and nnz_host; fill: cc, vc. */
+int tf_csr_interleave_count(int n, const int* ra, const int* ca, const int* rb, const int* cb, int gauge, int* cnt,
+                            int* rc, int* nnz_host, void* stream);
+int tf_csr_interleave_fill(int n, const int* ra, const int* ca, const double* va, const int* rb, const int* cb,
+                           const double* vb, int gauge, const int* rc, int* cc, double* vc, void* stream);
 /* diag[r] = A[r][r] of a CSR matrix, 0 for rows without a diagonal entry (the mass
  * matrix of Discretization.mass_matrix, Discretization.py:417-437, is diagonal). */
 int tf_csr_diagonal(int n, const int* row_ptr, const int* col, const double* val, double* diag, void* stream);

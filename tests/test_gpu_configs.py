@@ -150,3 +150,51 @@ def test_solve_multi_column_rhs(hip, precision):
     assert numpy.all(err <= 1e-8)
     with pytest.raises(ValueError):
         itf.solve(itf.jacobian(state), B[:-1])
+
+
+@pytest.mark.timeout(300)
+def test_rb_convective_branch_matches_oracle(hip):
+    """Rayleigh-Benard 64 x 16 (Pr 1, Biot 1, X-max 4) on the convective branch,
+    where the continuation does real corrector work: from the conduction state at
+    Ra = 5000 plus a seeded temperature perturbation, 150 backward-Euler steps
+    (dt = 1) grow the rolls, Newton converges them, and the continuation runs
+    Ra 5000 -> 6000 (ds 100, tol 1e-8).  The HIP backend (mixed precision) and
+    the oracle land on the same steady rolls and take the same steps."""
+    from oracle.interface import Interface as OracleInterface
+    from oracle.time_integration import TimeIntegration as OracleTI
+    from transiflow_amd.continuation import Continuation
+    params = dict(RB, **{'Rayleigh Number': 5000.0})
+    out = {}
+    for name in ('hip', 'oracle'):
+        if name == 'hip':
+            itf = hip.Interface(dict(params, **{'HIP Solver': {'Precision': 'mixed'}}), 64, 16)
+            ti = hip.TimeIntegration(itf, theta=1.0, newton_tolerance=1e-8, printer=None)
+            conv = hip.Continuation
+        else:
+            itf = OracleInterface(dict(params), 64, 16)
+            ti = OracleTI(itf, theta=1.0, newton_tolerance=1e-8)
+            conv = Continuation
+        n = itf.vector().size
+        cont = conv(itf, newton_tolerance=1e-8, printer=None)
+        x = cont.newton(itf.vector(), tol=1e-10)
+        h = numpy.zeros(n)
+        h[3::4] = 1e-2 * numpy.random.default_rng(1).uniform(-1, 1, n // 4)
+        x = x + (itf.vector_from_array(h) if name == 'hip' else h)
+        for _ in range(150):
+            x = ti.step(x, 1.0)
+        x = cont.newton(x, tol=1e-10)
+        xs = numpy.asarray(itf.array_from_vector(x) if name == 'hip' else x).copy()
+        mus, its = [], []
+        x, mu = cont.continuation(x, 'Rayleigh Number', 5000, 6000, 100,
+                                  callback=lambda _i, _x, m: (mus.append(m), its.append(cont.newton_iterations)))
+        out[name] = (xs, numpy.asarray(itf.array_from_vector(x) if name == 'hip' else x), mu, mus, its)
+    (sh, xh, mh, mush, itsh), (so, xo, mo, muso, itso) = out['hip'], out['oracle']
+    vel = numpy.arange(len(so)) % 4 < 2
+    e0 = numpy.linalg.norm(sh - so) / numpy.linalg.norm(so)
+    e1 = numpy.linalg.norm(xh - xo) / numpy.linalg.norm(xo)
+    print('convective rolls: max |u| %.3f, start rel diff %.2e; continuation Newton its %s (oracle %s), '
+          'final rel diff %.2e' % (numpy.abs(so[vel]).max(), e0, itsh, itso, e1), flush=True)
+    assert numpy.abs(so[vel]).max() > 0.05          # rolls, not the conduction state
+    assert e0 <= 1e-8 and e1 <= 1e-8
+    assert itsh == itso and sum(itso[1:]) > 0        # real corrector work
+    numpy.testing.assert_allclose(mush, muso, rtol=1e-9)

@@ -465,3 +465,30 @@ def test_fused_apply_matches_two_pass(hip, single, shape):
     diff = numpy.linalg.norm(out[1] - out[0]) / numpy.linalg.norm(out[0])
     print('%s %s: fused vs two-pass apply %.2e' % (shape, 'fp32' if single else 'fp64', diff), flush=True)
     assert diff <= (1e-3 if single else 1e-10)
+
+
+@pytest.mark.parametrize('alpha,beta', [(0.4 + 0.9j, 1.0), (-0.2 + 2.5j, 0.8 - 0.1j), (0.7, 1.0)])
+def test_shifted_complex_solve_matches_splu(hip, alpha, beta):
+    """Shifted solves of the eigenvalue glue (JaDa.py:98-99,146-151): the device
+    matrix beta J - alpha M with complex shifts, solved against the oracle's
+    complex splu of the same gauged matrix (SciPy.py:204-258)."""
+    from oracle.interface import Interface as OracleInterface
+    params = {'Reynolds Number': 300}
+    itf = hip.Interface(dict(params), 24, 16)
+    ref = OracleInterface(dict(params), 24, 16)
+    state = seeded_state(itf.n, 91) * 0.3
+    rng = numpy.random.default_rng(92)
+    b = rng.uniform(-1, 1, itf.n) + 1j * rng.uniform(-1, 1, itf.n)
+    A = beta * itf.jacobian(state) - alpha * itf.mass_matrix()
+    y = itf.solve(A, b)
+    Ao = beta * ref.jacobian(state) - alpha * ref.mass_matrix()
+    if numpy.iscomplexobj(Ao.data):
+        yo = ref.solve(Ao, b)
+    else:   # a real shift: the real factor solves the two parts (scipy's real LU takes no complex rhs)
+        yo = ref.solve(Ao, b.real) + 1j * ref.solve(Ao, b.imag)
+    err = numpy.linalg.norm(y - yo) / numpy.linalg.norm(yo)
+    its, rel, name, _ = itf.solver.log[-1]
+    print('alpha %s beta %s: %s, %d iterations, rel res %.2e, vs splu %.2e' % (alpha, beta, name, its, rel, err),
+          flush=True)
+    assert numpy.iscomplexobj(y) and y.shape == b.shape
+    assert rel <= 1e-10 and err <= 1e-8

@@ -48,6 +48,8 @@ SIGNATURES = {
     'tf_fill_f64': (_c_int, [_c_long, _c_double, _vp, _vp]),
     'tf_theta_residual': (_c_int, [_c_long, _vp, _vp, _vp, _vp, _vp, _c_double, _c_double, _vp, _vp]),
     'tf_csr_diagonal': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp]),
+    'tf_csr_interleave_count': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _ip, _vp]),
+    'tf_csr_interleave_fill': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp]),
     'tf_krylov_work_size': (_c_int, []),
     'tf_krylov_max_basis': (_c_int, []),
     'tf_multidot_f64': (_c_int, [_c_int, _c_int, _vp, _c_long, _vp, _vp, _vp, _c_int, _vp]),

@@ -120,6 +120,72 @@ __global__ __launch_bounds__(256) void k_theta_residual(long n, const double* __
     out[i] = Fv[i] + cb * Fu[i] + cm * (m[i] * (u[i] - v[i]));
 }
 
+// Real form of the complex matrix A + iB, interleaved: unknown c becomes the pair
+// (2c, 2c+1) = (re, im), row r the rows 2r (re) and 2r+1 (im):
+//   row 2r:   a (2c) and -b (2c+1);   row 2r+1:  b (2c) and a (2c+1)
+// over the union of the columns of row r in A and B.  Row and column ``gauge``
+// become -e (both halves), as the pressure gauge of the real solve.
+// count: cnt[r] = entries of each of the two output rows.
+__global__ __launch_bounds__(256) void k_interleave_count(int n, const int* __restrict__ ra, const int* __restrict__ ca,
+                                                          const int* __restrict__ rb, const int* __restrict__ cb,
+                                                          int gauge, int* __restrict__ cnt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if (r == gauge) {
+    cnt[2 * r] = cnt[2 * r + 1] = 1;
+    return;
+  }
+  int u = 0;
+  for (int p = ra[r]; p < ra[r + 1]; ++p) u += ca[p] != gauge;
+  for (int q = rb[r]; q < rb[r + 1]; ++q) {
+    if (cb[q] == gauge) continue;
+    bool found = false;
+    for (int p = ra[r]; p < ra[r + 1] && !found; ++p) found = ca[p] == cb[q];
+    u += !found;
+  }
+  cnt[2 * r] = cnt[2 * r + 1] = 2 * u;
+}
+
+__global__ __launch_bounds__(256) void k_interleave_fill(int n, const int* __restrict__ ra, const int* __restrict__ ca,
+                                                         const double* __restrict__ va, const int* __restrict__ rb,
+                                                         const int* __restrict__ cb, const double* __restrict__ vb,
+                                                         int gauge, const int* __restrict__ rc, int* __restrict__ cc,
+                                                         double* __restrict__ vc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int o0 = rc[2 * r], o1 = rc[2 * r + 1];
+  if (r == gauge) {
+    cc[o0] = 2 * r;
+    vc[o0] = -1.0;
+    cc[o1] = 2 * r + 1;
+    vc[o1] = -1.0;
+    return;
+  }
+  // columns of A (with the matching B value, if any), then B's columns missing in A
+  for (int p = ra[r]; p < ra[r + 1]; ++p) {
+    const int c = ca[p];
+    if (c == gauge) continue;
+    double b = 0.0;
+    for (int q = rb[r]; q < rb[r + 1]; ++q)
+      if (cb[q] == c) b = vb[q];
+    cc[o0] = 2 * c; vc[o0++] = va[p];
+    cc[o0] = 2 * c + 1; vc[o0++] = -b;
+    cc[o1] = 2 * c; vc[o1++] = b;
+    cc[o1] = 2 * c + 1; vc[o1++] = va[p];
+  }
+  for (int q = rb[r]; q < rb[r + 1]; ++q) {
+    const int c = cb[q];
+    if (c == gauge) continue;
+    bool found = false;
+    for (int p = ra[r]; p < ra[r + 1] && !found; ++p) found = ca[p] == c;
+    if (found) continue;
+    cc[o0] = 2 * c; vc[o0++] = 0.0;
+    cc[o0] = 2 * c + 1; vc[o0++] = -vb[q];
+    cc[o1] = 2 * c; vc[o1++] = vb[q];
+    cc[o1] = 2 * c + 1; vc[o1++] = 0.0;
+  }
+}
+
 // diag[r] = A[r][r] (0 when the row holds no diagonal entry)
 __global__ __launch_bounds__(256) void k_csr_diagonal(int n, const int* __restrict__ row_ptr,
                                                       const int* __restrict__ col, const double* __restrict__ val,
@@ -383,6 +449,29 @@ int tf_theta_residual(long n, const double* Fv, const double* Fu, const double* 
   hipLaunchKernelGGL(k_theta_residual, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, Fv, Fu, mdiag, u, v,
                      cb, cm, out);
   TF_LAUNCH_CHECK("k_theta_residual");
+  return 0;
+}
+
+int tf_csr_interleave_count(int n, const int* ra, const int* ca, const int* rb, const int* cb, int gauge, int* cnt,
+                            int* rc, int* nnz_host, void* stream) {
+  if (n < 1) return tf::fail("tf_csr_interleave_count: empty matrix");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_interleave_count, dim3((n + 255) / 256), dim3(256), 0, st, n, ra, ca, rb, cb, gauge, cnt);
+  hipLaunchKernelGGL(k_exclusive_scan_serial, dim3(1), dim3(1), 0, st, 2 * n, (const int*)cnt, rc);
+  TF_LAUNCH_CHECK("csr interleave count");
+  if (nnz_host) {
+    TF_HIP(hipMemcpyAsync(nnz_host, rc + 2 * n, sizeof(int), hipMemcpyDeviceToHost, st));
+    TF_HIP(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+
+int tf_csr_interleave_fill(int n, const int* ra, const int* ca, const double* va, const int* rb, const int* cb,
+                           const double* vb, int gauge, const int* rc, int* cc, double* vc, void* stream) {
+  if (n < 1) return tf::fail("tf_csr_interleave_fill: empty matrix");
+  hipLaunchKernelGGL(k_interleave_fill, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, ra, ca, va, rb, cb,
+                     vb, gauge, rc, cc, vc);
+  TF_LAUNCH_CHECK("csr interleave fill");
   return 0;
 }
 

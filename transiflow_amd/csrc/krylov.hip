@@ -22,45 +22,69 @@ constexpr int MD_THREADS = 256;
 constexpr int MD_BLOCKS = 256;   // fixed partition -> deterministic sums
 constexpr int MAX_BASIS = 256;
 
+constexpr int MD_WAVES = MD_THREADS / 64;
+constexpr int MD_KB = 8;   // basis vectors per pass over w
+
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, const T* __restrict__ V, long ldv,
                                                                  const T* __restrict__ w,
                                                                  double* __restrict__ partial) {
   // partial[i * MD_BLOCKS + b] = sum over block b's slice of V[i] . w, accumulated
-  // in the vector type (FP32 vectors: FP32 accumulation inside each block)
+  // in the vector type (FP32 vectors: FP32 accumulation inside each block).
+  // MD_KB basis vectors per pass, so w is read once per MD_KB vectors; sums are
+  // reduced by wavefront shuffles, then across the block's waves in a fixed order.
   using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
-  __shared__ Acc sh[MD_THREADS];
+  __shared__ Acc sh[MD_WAVES][MD_KB];
   const int chunk = (n + MD_BLOCKS - 1) / MD_BLOCKS;
   const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
-  for (int i = 0; i < k; ++i) {
-    const T* v = V + (long)i * ldv;
-    Acc acc = 0;
-    for (int e = b0 + threadIdx.x; e < b1; e += MD_THREADS) acc += (Acc)v[e] * (Acc)w[e];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = MD_THREADS / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-      __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i0 = 0; i0 < k; i0 += MD_KB) {
+    const int kb = min(MD_KB, k - i0);
+    Acc acc[MD_KB];
+#pragma unroll
+    for (int q = 0; q < MD_KB; ++q) acc[q] = 0;
+    for (int e = b0 + threadIdx.x; e < b1; e += MD_THREADS) {
+      const Acc we = (Acc)w[e];
+#pragma unroll
+      for (int q = 0; q < MD_KB; ++q)
+        if (q < kb) acc[q] += (Acc)V[(long)(i0 + q) * ldv + e] * we;
     }
-    if (threadIdx.x == 0) partial[i * MD_BLOCKS + blockIdx.x] = (double)sh[0];
+#pragma unroll
+    for (int q = 0; q < MD_KB; ++q) {
+      const Acc s = wave_sum(acc[q]);
+      if (lane == 0) sh[wave][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kb) {
+      Acc s = 0;
+      for (int v = 0; v < MD_WAVES; ++v) s += sh[v][threadIdx.x];
+      partial[(long)(i0 + threadIdx.x) * MD_BLOCKS + blockIdx.x] = (double)s;
+    }
     __syncthreads();
   }
 }
 
-// h[i] (+)= sum_b partial[i][b]; one workgroup per i.
+// h[i] (+)= sum_b partial[i][b]; one workgroup per i, wavefront reductions.
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, const double* __restrict__ partial,
                                                                double* __restrict__ h, int accumulate) {
-  __shared__ double sh[MD_THREADS];
+  __shared__ double sh[MD_WAVES];
   const int i = blockIdx.x;
   double acc = 0.0;
-  for (int b = threadIdx.x; b < MD_BLOCKS; b += MD_THREADS) acc += partial[i * MD_BLOCKS + b];
-  sh[threadIdx.x] = acc;
+  for (int b = threadIdx.x; b < MD_BLOCKS; b += MD_THREADS) acc += partial[(long)i * MD_BLOCKS + b];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int s = MD_THREADS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int v = 0; v < MD_WAVES; ++v) s += sh[v];
+    h[i] = accumulate ? h[i] + s : s;
   }
-  if (threadIdx.x == 0) h[i] = accumulate ? h[i] + sh[0] : sh[0];
 }
 
 // w -= sum_i coef[i] V[i]   (coef on the device, FP64)
@@ -89,16 +113,17 @@ __global__ __launch_bounds__(256) void k_scale_by_inv(int n, const T* __restrict
 // sqrt(sum partial) into out
 __global__ __launch_bounds__(MD_THREADS) void k_norm_final(const double* __restrict__ partial,
                                                            double* __restrict__ out) {
-  __shared__ double sh[MD_THREADS];
+  __shared__ double sh[MD_WAVES];
   double acc = 0.0;
   for (int b = threadIdx.x; b < MD_BLOCKS; b += MD_THREADS) acc += partial[b];
-  sh[threadIdx.x] = acc;
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int s = MD_THREADS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int v = 0; v < MD_WAVES; ++v) s += sh[v];
+    *out = sqrt(s);
   }
-  if (threadIdx.x == 0) *out = sqrt(sh[0]);
 }
 
 // ---------------------------------------------------------------------------

@@ -94,7 +94,7 @@ import weakref
 import numpy
 
 from . import _lib
-from .vector import DeviceVector, ptr, stream, torch
+from .vector import ComplexCsrMatrix, DeviceCsrMatrix, DeviceVector, ptr, stream, torch
 
 LINE_BUDGET_BYTES = 64 * 2 ** 30
 INNER_FLOOR = 1e-5    # FP32 inner solves stall near this relative residual on these Jacobians
@@ -186,13 +186,18 @@ class LineFactor:
     Thomas recursion (``'thomas'``, csrc/linelu.hip)."""
     _ABI = {'line': 'tf_bcr', 'thomas': 'tf_linelu'}
 
-    def __init__(self, itf, jac, gauge, single=False, kind='line', factor=True):
+    def __init__(self, itf, jac, gauge, single=False, kind='line', factor=True, layout=None):
+        """``layout`` (n, m, nx, dof, ylines) overrides the problem's own (the
+        interleaved real form of a complex matrix has twice the rows per line)."""
         self.name = kind
         self._p = self._ABI[kind]
         h = ctypes.c_void_p()
         if kind == 'line':
             m, ylines = line_layout(itf)
-            _lib.call('tf_bcr_create', ctypes.byref(h), itf.n, m, 1 if single else 0, itf.nx, itf.dof, ylines)
+            n, nx, dof = itf.n, itf.nx, itf.dof
+            if layout is not None:
+                n, m, nx, dof, ylines = layout
+            _lib.call('tf_bcr_create', ctypes.byref(h), n, m, 1 if single else 0, nx, dof, ylines)
         else:
             m = line_block_size(itf)
             _lib.call(self._p + '_create', ctypes.byref(h), itf.n, m, 1 if single else 0)
@@ -828,6 +833,8 @@ class Solver:
         an (n, k) array gives an (n, k) numpy array whose columns are solved
         one after another, each with its ``pressure_row`` entry zeroed
         (SciPy.py:209-216, where splu solves all columns of the gauged rhs)."""
+        if isinstance(jac, ComplexCsrMatrix):
+            return self.solve_complex(jac, rhs)
         if rhs2 is not None or V is not None:
             return self.solve_bordered(jac, rhs, rhs2, V, W, C)
         n = self.itf.n
@@ -835,6 +842,12 @@ class Solver:
             if rhs.size != n:
                 raise ValueError('rhs has %d entries, the system has %d rows' % (rhs.size, n))
             return DeviceVector(self._solve_device(jac, rhs.data, zero_gauge=True))
+        if numpy.iscomplexobj(rhs):
+            # a real matrix and a complex rhs: the real and imaginary parts apart
+            c = numpy.asarray(rhs)
+            parts = [self.solve(jac, numpy.ascontiguousarray(p)) for p in (c.real, c.imag)]
+            parts = [numpy.asarray(p.numpy() if isinstance(p, DeviceVector) else p) for p in parts]
+            return parts[0] + 1j * parts[1]
         a = numpy.asarray(rhs, dtype=numpy.float64)
         if a.ndim == 1 and a.shape[0] == n:
             return DeviceVector(self._solve_device(jac, DeviceVector.from_array(a).data, zero_gauge=True))
@@ -1048,6 +1061,123 @@ class Solver:
             rel = _norm(r, work) / bnrm
         self.refinements.append(outer)
         return total, rel, prec, fresh
+
+    def solve_complex(self, A, rhs):
+        """y with (gauged A) y = rhs for a complex matrix A = Ar + i Ai (the shifted
+        beta J - alpha M of the eigenvalue glue, JaDa.py:146-151, which the
+        reference solves with a complex splu, SciPy.py:204-258): row and column
+        ``pressure_row`` of A become -e and the rhs's ``pressure_row`` entry is
+        zeroed.
+
+        Solved in real arithmetic on the interleaved form K (unknown c becomes
+        the pair (re, im), ``tf_csr_interleave_*``): K keeps the grid-line block
+        structure with twice the rows per line, so the exact line factorisation
+        (block cyclic reduction) factors it in FP64 and FGMRES needs one or two
+        iterations.  When that factor does not fit the memory budget the solve
+        runs on [[Ar, -Ai], [Ai, Ar]] preconditioned block-diagonally by the
+        factor of Ar (slower as |Ai| grows).  rhs of shape (n,) or (n, k);
+        returns a complex numpy array of the same shape."""
+        T = torch()
+        n = self.itf.n
+        b = numpy.asarray(rhs.numpy() if isinstance(rhs, DeviceVector) else rhs)
+        if b.ndim == 2 and b.shape[0] == n:
+            return numpy.stack([self.solve_complex(A, b[:, j]) for j in range(b.shape[1])], axis=1)
+        if b.ndim != 1 or b.shape[0] != n:
+            raise ValueError('rhs must have shape (%d,) or (%d, k), got %s' % (n, n, b.shape))
+        opts = _opts(self.itf.parameters)
+        g = self.gauge()
+        b = numpy.ascontiguousarray(b, dtype=numpy.complex128)
+        if g >= 0:
+            b[g] = 0.0
+        setup = getattr(A, '_real_form', None)    # made once per matrix (the glue reuses its shifted matrices)
+        if setup is None:
+            setup = A._real_form = self._real_form(A)
+        K, factor = setup
+        if factor is None:
+            return self._solve_complex_blockdiag(A, b, opts)
+        b2 = T.from_numpy(b.view(numpy.float64).copy()).cuda()   # (re, im) pairs: the interleaved layout
+        x2 = T.zeros(2 * n, dtype=T.float64, device='cuda')
+        kry = getattr(self, '_complex_krylov', None)
+        if kry is None or kry.m != opts['Restart'] or kry.n != 2 * n:
+            kry = self._complex_krylov = NativeKrylov(2 * n, opts['Restart'])
+        its, rel = kry.solve(K, K.val, -1, factor, b2, x2, opts['Tolerance'], opts['Maximum Iterations'],
+                             x_is_zero=True)
+        self.log.append((its, rel, 'complex-line', False))
+        if not rel <= opts['Tolerance'] and opts['Fail On Tolerance']:
+            raise SolverError('complex GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
+                              % (rel, its))
+        return x2.cpu().numpy().view(numpy.complex128).copy()
+
+    def _real_form(self, A):
+        """(K, factor): the interleaved real form of A with the gauge applied, and
+        its exact FP64 line factorisation (None when it does not fit)."""
+        T = torch()
+        itf = self.itf
+        n = itf.n
+        g = self.gauge()
+        st = stream()
+        cnt = T.empty(2 * n, dtype=T.int32, device='cuda')
+        rc = T.empty(2 * n + 1, dtype=T.int32, device='cuda')
+        nnz = ctypes.c_int()
+        Ar, Ai = A.real, A.imag
+        _lib.call('tf_csr_interleave_count', n, ptr(Ar.row_ptr), ptr(Ar.col), ptr(Ai.row_ptr), ptr(Ai.col), int(g),
+                  ptr(cnt), ptr(rc), ctypes.byref(nnz), st)
+        cc = T.empty(max(nnz.value, 1), dtype=T.int32, device='cuda')[:nnz.value]
+        vc = T.empty(max(nnz.value, 1), dtype=T.float64, device='cuda')[:nnz.value]
+        _lib.call('tf_csr_interleave_fill', n, ptr(Ar.row_ptr), ptr(Ar.col), ptr(Ar.val), ptr(Ai.row_ptr),
+                  ptr(Ai.col), ptr(Ai.val), int(g), ptr(rc), ptr(cc), ptr(vc), st)
+        K = DeviceCsrMatrix(rc, cc, vc, 2 * n)
+        m, ylines = line_layout(itf)
+        if 5 * (n // m) * (2 * m) ** 2 * 8 > LINE_BUDGET_BYTES:
+            return K, None
+        f = LineFactor(itf, K, -1, single=False, kind='line', factor=False,
+                       layout=(2 * n, 2 * m, itf.nx, 2 * itf.dof, ylines))
+        f.set_graph(_opts(itf.parameters)['Apply Graph'])
+        f.refactor(K, -1)
+        return K, f
+
+    def _solve_complex_blockdiag(self, A, b, opts):
+        """[[Ar, -Ai], [Ai, Ar]] by FGMRES with the factor of Ar on both halves."""
+        T = torch()
+        n = self.itf.n
+        g = self.gauge()
+        st = stream()
+        Ar, B = A.real, A.imag
+        prec = getattr(Ar, 'prec64', None)
+        if prec is None or prec.owner is None or prec.owner() is not Ar:
+            prec = Ar.prec64 = self._build(Ar, opts['Preconditioner'], single=False)
+        tmp = T.empty(n, dtype=T.float64, device='cuda')
+
+        def bmul(v, out):
+            B.matvec(v, out=out, gauge=g)
+            if g >= 0:
+                _lib.call('tf_set_entry_f64', ptr(out), n, g, 0.0, st)   # the gauge row of Ai is zero
+
+        def matvec(v, out):
+            vr, vi = v[:n], v[n:]
+            Ar.matvec(vr, out=out[:n], gauge=g)
+            bmul(vi, tmp)
+            _lib.call('tf_axpby_f64', n, 1.0, ptr(out[:n]), -1.0, ptr(tmp), ptr(out[:n]), st)
+            Ar.matvec(vi, out=out[n:], gauge=g)
+            bmul(vr, tmp)
+            _lib.call('tf_axpby_f64', n, 1.0, ptr(out[n:]), 1.0, ptr(tmp), ptr(out[n:]), st)
+
+        def precond(v, out):
+            prec.apply(v[:n], out[:n])
+            prec.apply(v[n:], out[n:])
+
+        b2 = T.from_numpy(numpy.concatenate([b.real, b.imag])).cuda()
+        x2 = T.zeros(2 * n, dtype=T.float64, device='cuda')
+        kry = getattr(self, '_complex_krylov_bd', None)
+        if kry is None or kry.m != opts['Restart'] or kry.n != 2 * n:
+            kry = self._complex_krylov_bd = Krylov(2 * n, opts['Restart'])
+        its, rel = kry.solve(matvec, precond, b2, x2, opts['Tolerance'], opts['Maximum Iterations'], x_is_zero=True)
+        self.log.append((its, rel, 'complex+' + getattr(prec, 'name', '?'), True))
+        if not rel <= opts['Tolerance'] and opts['Fail On Tolerance']:
+            raise SolverError('complex GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
+                              % (rel, its))
+        x = x2.cpu().numpy()
+        return x[:n] + 1j * x[n:]
 
     def line_stats(self):
         """Shape of the line factorisation for this problem (rows per line, lines)."""

@@ -290,24 +290,102 @@ class DeviceCsrMatrix:
     def __add__(self, other):
         if isinstance(other, DeviceCsrMatrix):
             return self._combine(1.0, other, 1.0)
+        if isinstance(other, ComplexCsrMatrix):
+            return ComplexCsrMatrix(self._combine(1.0, other.real, 1.0), other.imag)
         return NotImplemented
 
     def __sub__(self, other):
         if isinstance(other, DeviceCsrMatrix):
             return self._combine(1.0, other, -1.0)
+        if isinstance(other, ComplexCsrMatrix):
+            return ComplexCsrMatrix(self._combine(1.0, other.real, -1.0), -other.imag)
         return NotImplemented
 
     def __mul__(self, alpha):
-        if isinstance(alpha, numbers.Number):
+        if isinstance(alpha, numbers.Real):
             return self._scaled(alpha)
+        if isinstance(alpha, numbers.Complex):
+            a = complex(alpha)
+            return ComplexCsrMatrix(self._scaled(a.real), self._scaled(a.imag))
         return NotImplemented
 
     __rmul__ = __mul__
 
     def __truediv__(self, alpha):
         if isinstance(alpha, numbers.Number):
-            return self._scaled(1.0 / alpha)
+            return self * (1.0 / alpha)
         return NotImplemented
 
     def __neg__(self):
         return self._scaled(-1.0)
+
+
+class ComplexCsrMatrix:
+    """A + i B for real device CSR matrices A, B of one pattern size: what the
+    scipy arithmetic of the reference's eigenvalue glue builds from complex
+    shifts, ``beta * J - alpha * M`` (JaDa.py:98-99), and what
+    ``Interface.solve`` then solves (in the interleaved real form)."""
+
+    def __init__(self, real, imag):
+        self.real, self.imag = real, imag
+        self.n = real.n
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    @property
+    def dtype(self):
+        return numpy.dtype(numpy.complex128)
+
+    @staticmethod
+    def _parts(other):
+        if isinstance(other, ComplexCsrMatrix):
+            return other.real, other.imag
+        if isinstance(other, DeviceCsrMatrix):
+            return other, None
+        return None
+
+    def _combine(self, other, sign):
+        p = self._parts(other)
+        if p is None:
+            return NotImplemented
+        re = self.real._combine(1.0, p[0], sign)
+        im = self.imag if p[1] is None else self.imag._combine(1.0, p[1], sign)
+        return ComplexCsrMatrix(re, im)
+
+    def __add__(self, other):
+        return self._combine(other, 1.0)
+
+    def __sub__(self, other):
+        return self._combine(other, -1.0)
+
+    def __radd__(self, other):
+        return self._combine(other, 1.0)
+
+    def __rsub__(self, other):
+        p = self._parts(other)
+        if p is None:
+            return NotImplemented
+        return (-self)._combine(other, 1.0)
+
+    def __neg__(self):
+        return ComplexCsrMatrix(-self.real, -self.imag)
+
+    def __mul__(self, alpha):
+        if not isinstance(alpha, numbers.Number):
+            return NotImplemented
+        a = complex(alpha)
+        # (A + iB)(a + ib) = (a A - b B) + i (b A + a B)
+        return ComplexCsrMatrix(self.real._combine(a.real, self.imag, -a.imag),
+                                self.real._combine(a.imag, self.imag, a.real))
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, alpha):
+        if isinstance(alpha, numbers.Number):
+            return self * (1.0 / alpha)
+        return NotImplemented
+
+    def to_scipy(self):
+        return self.real.to_scipy() + 1j * self.imag.to_scipy()
