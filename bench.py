@@ -76,12 +76,18 @@ def workload(name, n, target, ds):
                          'the steady rolls at Ra=5000 (untimed set-up: backward-Euler steps dt=1 from the conduction '
                          'state plus a 1e-2*U(-1,1) (default_rng(1)) temperature perturbation until the velocity '
                          'saturates, then Newton), continuation Ra 5000->8000, ds=100, tol 1e-8'}
-    if name in ('ldc3d', 'ldc3d64'):
-        k = 32 if name == 'ldc3d' else 64
-        return {'params': {}, 'shape': (k, k, k), 'param': 'Reynolds Number', 'target': 1000.0, 'ds': 100.0,
+    if name == 'ldc3d':
+        return {'params': {}, 'shape': (32, 32, 32), 'param': 'Reynolds Number', 'target': 1000.0, 'ds': 100.0,
                 'perturb': False,
-                'label': 'lid-driven cavity 3D %d^3 (n=%d), Newton from zero at Re=0, continuation Re 0->1000, '
-                         'ds=100, tol 1e-8' % (k, 4 * k ** 3)}
+                'label': 'lid-driven cavity 3D 32^3 (n=131072), Newton from zero at Re=0, continuation Re 0->1000, '
+                         'ds=100, tol 1e-8'}
+    if name == 'ldc3d64':
+        # the two-grid preconditioner converges in at most ~60 iterations up to Re 650 at
+        # this size, needs 200-400 at Re 800 and stalls near Re 900
+        return {'params': {}, 'shape': (64, 64, 64), 'param': 'Reynolds Number', 'target': 600.0, 'ds': 100.0,
+                'perturb': False, 'solver': {'Restart': 100, 'Maximum Iterations': 600},
+                'label': 'lid-driven cavity 3D 64^3 (n=1048576), Newton from zero at Re=0, continuation Re 0->600, '
+                         'ds=100, tol 1e-8, two-grid preconditioner (restart 100)'}
     return {'params': {}, 'shape': (n, n, 1), 'param': 'Reynolds Number', 'target': target, 'ds': ds,
             'perturb': False,
             'label': 'lid-driven cavity 2D %dx%d (n=%d), Newton from zero at Re=0, pseudo-arclength continuation '
@@ -96,6 +102,7 @@ VARIANTS = {
     'learned16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},   # + the trained network
     'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
     'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
+    'mixed16eager': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Step Graphs': False},
     'mixed16bg': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True},
     'mixed16s3': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16x3'},
     'mixed16s1': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16'},
@@ -108,6 +115,8 @@ DTYPES = {
     'mixed': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor)',
     'mixed16': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies '
                'of the factor blocks with f32 row scales)',
+    'mixed16eager': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES with eager Arnoldi launches, line factor; '
+                    'apply reads f16 copies of the factor blocks with f32 row scales)',
     'mixed16bg': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor factored on a side stream; '
                  'apply reads f16 copies of the factor blocks with f32 row scales)',
     'mixed16s3': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor with its level GEMMs on '
@@ -186,6 +195,7 @@ def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     import numpy
     params = dict(spec['params'])
     params['HIP Solver'] = dict(VARIANTS[variant], Restart=restart)
+    params['HIP Solver'].update(spec.get('solver', {}))
     if variant == 'learned16':
         params['HIP Solver']['Learned Network'] = LEARNED['net']
     itf = tf.Interface(params, *spec['shape'])
@@ -345,7 +355,8 @@ def vector_rooflines(torch, itf, x, reps=50, j=15):
         'gmres_arnoldi_vector_ops_f32': {'basis': j + 1, 'bytes': ba, 'us': ta * 1e6, 'GBps': ba / ta / 1e9,
                                          'frac_hbm_peak': ba / ta / 1e9 / HBM_PEAK_GBS},
         'n': n, 'nnz': nnz,
-        'note': 'working sets of this size sit in the 256 MB Infinity Cache; rates are algorithmic bytes / time',
+        'note': 'rates are algorithmic bytes / time; the FP64 SpMV working set is %.0f MB against the 256 MB '
+                'Infinity Cache' % (b64 / 1e6),
     }
 
 
@@ -453,6 +464,8 @@ def main():
                     help="comma-separated solver variants (default: 'fp64,fp64reuse,mixed16,learned16' for the 2D "
                          "cavity, 'fp64,mixed16' for the other workloads)")
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-hbm-sizes', action='store_true',
+                    help='skip the assembly and SpMV rates at 1024^2 and 64^3 (working sets beyond the cache)')
     ap.add_argument('--workload', default='ldc', choices=['ldc', 'batch', 'rb', 'rbconv', 'ldc3d', 'ldc3d64'],
                     help="'ldc': one 256x256 cavity continuation (default); 'batch': 64 seeded cavity "
                          "instances of uneven size and target Reynolds number; 'rb': Rayleigh-Benard 2D "
@@ -516,6 +529,17 @@ def main():
     log('kernel timing sample')
     asm = assembly_roofline(tf, torch, hr['itf'], hr['x'])
     vec = vector_rooflines(torch, hr['itf'], hr['x'])
+    hbm_sizes = {}
+    if args.workload == 'ldc' and not args.no_hbm_sizes:
+        for label, shape in (('LDC 2D 1024x1024, Re 1000', (1024, 1024, 1)), ('LDC 3D 64^3, Re 1000', (64, 64, 64))):
+            log('assembly and SpMV at %s' % label)
+            itf = tf.Interface({'Reynolds Number': 1000.0}, *shape)
+            rng = numpy.random.default_rng(3)
+            xs = itf.vector_from_array(rng.uniform(-0.1, 0.1, itf.n))
+            hbm_sizes[label] = {'assembly': assembly_roofline(tf, torch, itf, xs, reps=10),
+                                'spmv_gmres': vector_rooflines(torch, itf, xs, reps=20)}
+            del itf, xs
+            torch.cuda.empty_cache()
     log('profiled repeat: %s' % head)
     pr, kt = profiled_run(tf, torch, spec, head, maxit)
     pid, (calls, ms, work) = max(kt.items(), key=lambda kv: kv[1][1])
@@ -584,6 +608,7 @@ def main():
         'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},
         'assembly': asm,
         'spmv_gmres': vec,
+        'assembly_spmv_at_hbm_sizes': hbm_sizes,
         'roofline': roof,
         'cpu_baseline': cpu,
     }

@@ -158,6 +158,10 @@ int tf_normalize_f32(int n, const float* w, double* work, double* nrm_dev, float
 typedef struct tf_krylov tf_krylov;
 int tf_krylov_create(tf_krylov** out, int n, int restart, int single);
 int tf_krylov_destroy(tf_krylov* k);
+/* on != 0 (the default): each Arnoldi step (SpMV through a device matrix descriptor, two
+ * CGS passes, normalisation, the read-back of the Hessenberg column) is captured once per
+ * basis index into a hipGraph and replayed; 0: the same launches are issued one by one. */
+int tf_krylov_set_graph(tf_krylov* k, int on);
 int tf_fgmres(tf_krylov* k, const int* row_ptr, const int* col, const void* val, int gauge, int prec_kind,
               void* prec, int ncell, int dof, const void* b, void* x, double tol, int maxit, int x_is_zero,
               int* its_out, double* rel_out, void* stream);

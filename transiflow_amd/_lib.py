@@ -80,6 +80,7 @@ SIGNATURES = {
     'tf_to_bf16': (_c_int, [_c_long, _vp, ctypes.c_float, _vp, _vp]),
     'tf_krylov_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
     'tf_krylov_destroy': (_c_int, [_vp]),
+    'tf_krylov_set_graph': (_c_int, [_vp, _c_int]),
     'tf_fgmres': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_double, _c_int,
                            _c_int, _ip, _dp, _vp]),
     'tf_csr_add_count': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _vp]),

@@ -3,6 +3,7 @@
 // Reductions are deterministic: a fixed grid of TF_RED_BLOCKS workgroups each
 // reduces a fixed contiguous slice in a fixed tree order, then one workgroup
 // reduces the partials; the same inputs give the same bits on every run.
+#include "spmv.h"
 #include "tf_common.h"
 
 #include <mutex>
@@ -243,20 +244,7 @@ template <typename TV, typename TX>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                               const TV* __restrict__ v, const TX* __restrict__ x, TX* __restrict__ y,
                                               int gauge) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  if (row == gauge) {
-    y[row] = -x[row];
-    return;
-  }
-  using Acc = typename std::conditional<sizeof(TX) == 8, double, float>::type;
-  Acc acc = 0;
-  const int e = rp[row + 1];
-  for (int k = rp[row]; k < e; ++k) {
-    const int c = ci[k];
-    if (c != gauge) acc += (Acc)v[k] * (Acc)x[c];
-  }
-  y[row] = (TX)acc;
+  tf::spmv_rows<tf::SPMV_LANES>(n, rp, ci, v, x, y, gauge);
 }
 
 __global__ void k_set_entry(double* x, long i, double v) { x[i] = v; }
@@ -565,8 +553,8 @@ int tf_dot_f64(long n, const double* x, const double* y, double* work, double* o
 int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const double* x, double* y, int gauge,
                 void* stream) {
   TF_PROF_BEGIN(tf::PROF_SPMV, (hipStream_t)stream);
-  hipLaunchKernelGGL((k_spmv<double, double>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
-                     x, y, gauge);
+  hipLaunchKernelGGL((k_spmv<double, double>), dim3(tf::spmv_blocks(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci,
+                     v, x, y, gauge);
   TF_PROF_END(tf::PROF_SPMV, (hipStream_t)stream);
   TF_LAUNCH_CHECK("k_spmv f64");
   return 0;
@@ -574,8 +562,8 @@ int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const doub
 
 int tf_spmv_f32(int n, const int* rp, const int* ci, const float* v, const float* x, float* y, int gauge,
                 void* stream) {
-  hipLaunchKernelGGL((k_spmv<float, float>), dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v, x,
-                     y, gauge);
+  hipLaunchKernelGGL((k_spmv<float, float>), dim3(tf::spmv_blocks(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
+                     x, y, gauge);
   TF_LAUNCH_CHECK("k_spmv f32");
   return 0;
 }

@@ -8,11 +8,31 @@
 // reference's scipy.sparse.linalg.gmres use (SciPy.py:343-370) restated for
 // FGMRES: stop when the residual estimate |g_{j+1}| <= tol ||b||, restart with
 // the true residual b - A x.
+//
+// The device part of Arnoldi step j (SpMV of Z[j], two Gram-Schmidt passes
+// against V[0..j], the normalisation into V[j+1] and the copy of the (j+2)-entry
+// column to pinned host memory) is captured once per j into a hipGraph and
+// replayed with one launch; the SpMV reads the matrix through a small device
+// descriptor that each solve points at its own CSR arrays, so the graphs stay
+// valid across Jacobians.  The host then waits for the column by polling the
+// stream (no blocking wait, whose wake-up would leave the GPU idle longer).
+#include "spmv.h"
 #include "tf_common.h"
 #include "transiflow_hip.h"
 
 #include <cmath>
 #include <vector>
+
+namespace {
+constexpr int MAX_STEPS = 256;   // Arnoldi steps per restart cycle (tf_krylov_max_basis)
+
+struct MatrixDesc {
+  const int* rp;
+  const int* ci;
+  const void* val;
+  int gauge;
+};
+}  // namespace
 
 struct tf_krylov {
   int n, m, single;
@@ -24,9 +44,27 @@ struct tf_krylov {
   double* y;   // m+1 (device)
   double* work;
   double* h_host;  // m+2 (pinned)
+  MatrixDesc* desc;          // the matrix of the current solve (device)
+  hipStream_t cap;           // private stream the step graphs are captured on
+  hipGraphExec_t step[MAX_STEPS];
+  hipGraph_t step_graph[MAX_STEPS];
+  int use_graph;
 };
 
 namespace {
+
+__global__ void k_set_desc(MatrixDesc* d, const int* rp, const int* ci, const void* val, int gauge) {
+  d->rp = rp;
+  d->ci = ci;
+  d->val = val;
+  d->gauge = gauge;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_desc(int n, const MatrixDesc* __restrict__ d, const T* __restrict__ x,
+                                                   T* __restrict__ y) {
+  tf::spmv_rows<tf::SPMV_LANES>(n, d->rp, d->ci, (const T*)d->val, x, y, d->gauge);
+}
 
 struct Ops {
   const tf_krylov* k;
@@ -80,6 +118,77 @@ int norm(const tf_krylov* k, int n, const void* x, double* out, hipStream_t st) 
   return 0;
 }
 
+// The device part of Arnoldi step j on stream st: w = A Z[j] (through the
+// descriptor), CGS2 h = V^T w, w -= V h, y = V^T w, w -= V y, h += y, V[j+1] =
+// w / |w| with |w| into h[j+1], and h[0..j+1] copied to h_host.
+int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
+  const int n = k->n;
+  Ops o{k, n, st};
+  if (k->single)
+    hipLaunchKernelGGL(k_spmv_desc<float>, dim3(tf::spmv_blocks(n)), dim3(256), 0, st, n, k->desc,
+                       (const float*)o.Z(j), (float*)k->w);
+  else
+    hipLaunchKernelGGL(k_spmv_desc<double>, dim3(tf::spmv_blocks(n)), dim3(256), 0, st, n, k->desc,
+                       (const double*)o.Z(j), (double*)k->w);
+  TF_LAUNCH_CHECK("fgmres spmv");
+  if (k->single) {
+    const float* Vf = (const float*)k->V;
+    float* wf = (float*)k->w;
+    if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->h, 0, st)) return e;
+    if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->h, wf, -1.0, st)) return e;
+    if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->y, 0, st)) return e;
+    if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->y, wf, -1.0, st)) return e;
+  } else {
+    const double* Vd = (const double*)k->V;
+    double* wd = (double*)k->w;
+    if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->h, 0, st)) return e;
+    if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->h, wd, -1.0, st)) return e;
+    if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->y, 0, st)) return e;
+    if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->y, wd, -1.0, st)) return e;
+  }
+  if (int e = tf_axpby_f64(j + 1, 1.0, k->h, 1.0, k->y, k->h, st)) return e;
+  if (int e = k->single ? tf_normalize_f32(n, (const float*)k->w, k->work, k->h + j + 1, (float*)o.V(j + 1), st)
+                        : tf_normalize_f64(n, (const double*)k->w, k->work, k->h + j + 1, (double*)o.V(j + 1), st))
+    return e;
+  TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+// Step j replayed from its captured graph (captured on first use).
+int arnoldi_step_graph(tf_krylov* k, int j, hipStream_t st) {
+  if (!k->step[j]) {
+    if (!k->cap) TF_HIP(hipStreamCreateWithFlags(&k->cap, hipStreamNonBlocking));
+    TF_HIP(hipStreamBeginCapture(k->cap, hipStreamCaptureModeThreadLocal));
+    const int e = arnoldi_step(k, j, k->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(k->cap, &g);
+    if (e) {
+      if (g) (void)hipGraphDestroy(g);
+      return e;
+    }
+    TF_HIP(ce);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return tf::fail_hip(ie, "hipGraphInstantiate (Arnoldi step)");
+    }
+    k->step_graph[j] = g;
+    k->step[j] = ex;
+  }
+  TF_HIP(hipGraphLaunch(k->step[j], st));
+  return 0;
+}
+
+// Wait for stream st by polling: returns once everything queued has finished.
+int poll_wait(hipStream_t st) {
+  for (;;) {
+    const hipError_t q = hipStreamQuery(st);
+    if (q == hipSuccess) return 0;
+    if (q != hipErrorNotReady) return tf::fail_hip(q, "hipStreamQuery");
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,8 +210,10 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
     TF_HIP(hipMalloc(&k->y, sizeof(double) * (restart + 1)));
     TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
     TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
+    TF_HIP(hipMalloc(&k->desc, sizeof(MatrixDesc)));
     return 0;
   };
+  k->use_graph = 1;
   if (int e = alloc()) {
     tf_krylov_destroy(k);  // frees the partial set (pointers start null)
     return e;
@@ -111,9 +222,20 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
   return 0;
 }
 
+int tf_krylov_set_graph(tf_krylov* k, int on) {
+  if (!k) return tf::fail("null handle");
+  k->use_graph = on ? 1 : 0;
+  return 0;
+}
+
 int tf_krylov_destroy(tf_krylov* k) {
   if (!k) return 0;
-  for (void* p : {k->V, k->Z, k->w, (void*)k->h, (void*)k->y, (void*)k->work}) (void)hipFree(p);
+  for (int j = 0; j < MAX_STEPS; ++j) {
+    if (k->step[j]) (void)hipGraphExecDestroy(k->step[j]);
+    if (k->step_graph[j]) (void)hipGraphDestroy(k->step_graph[j]);
+  }
+  if (k->cap) (void)hipStreamDestroy(k->cap);
+  for (void* p : {k->V, k->Z, k->w, (void*)k->h, (void*)k->y, (void*)k->work, (void*)k->desc}) (void)hipFree(p);
   (void)hipHostFree(k->h_host);
   delete k;
   return 0;
@@ -139,6 +261,9 @@ int tf_fgmres(tf_krylov* k, const int* rp, const int* ci, const void* val, int g
   if (bnrm == 0.0)
     return k->single ? tf_fill_f32(n, 0.0, (float*)x, st) : tf_fill_f64(n, 0.0, (double*)x, st);
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yk(m);
+  // the Arnoldi step graphs read the matrix through the descriptor
+  hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, st, k->desc, rp, ci, val, gauge);
+  TF_LAUNCH_CHECK("fgmres descriptor");
   auto Hij = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
   int its = 0;
   double rel = INFINITY;
@@ -168,30 +293,8 @@ int tf_fgmres(tf_krylov* k, const int* rp, const int* ci, const void* val, int g
     bool estimate_exit = false;
     for (int j = 0; j < m; ++j) {
       if (int e = precond(k, prec_kind, prec, ncell, dof, o.V(j), o.Z(j), st)) return e;
-      if (int e = spmv(k, n, rp, ci, val, gauge, o.Z(j), k->w, st)) return e;
-      // CGS2: h = V^T w, w -= V h, y = V^T w, w -= V y, h += y
-      if (k->single) {
-        const float* Vf = (const float*)k->V;
-        float* wf = (float*)k->w;
-        if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->h, 0, st)) return e;
-        if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->h, wf, -1.0, st)) return e;
-        if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->y, 0, st)) return e;
-        if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->y, wf, -1.0, st)) return e;
-      } else {
-        const double* Vd = (const double*)k->V;
-        double* wd = (double*)k->w;
-        if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->h, 0, st)) return e;
-        if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->h, wd, -1.0, st)) return e;
-        if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->y, 0, st)) return e;
-        if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->y, wd, -1.0, st)) return e;
-      }
-      if (int e = tf_axpby_f64(j + 1, 1.0, k->h, 1.0, k->y, k->h, st)) return e;
-      if (int e = k->single ? tf_normalize_f32(n, (const float*)k->w, k->work, k->h + j + 1, (float*)o.V(j + 1), st)
-                            : tf_normalize_f64(n, (const double*)k->w, k->work, k->h + j + 1, (double*)o.V(j + 1),
-                                               st))
-        return e;
-      TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
-      TF_HIP(hipStreamSynchronize(st));
+      if (int e = (k->use_graph && !tf::prof_on()) ? arnoldi_step_graph(k, j, st) : arnoldi_step(k, j, st)) return e;
+      if (int e = poll_wait(st)) return e;
       for (int i = 0; i <= j + 1; ++i) Hij(i, j) = k->h_host[i];
       for (int i = 0; i < j; ++i) {
         const double t = cs[i] * Hij(i, j) + sn[i] * Hij(i + 1, j);

@@ -35,6 +35,9 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
 ``'Apply Graph'``      replay the line-factor apply as one captured hipGraph
                        (True);
+``'Step Graphs'``      replay each native FGMRES Arnoldi step (SpMV, two CGS
+                       passes, normalisation) as one captured hipGraph (True)
+                       or launch its kernels one by one;
 ``'Apply Storage'``    (mixed, line factor) ``'fp16'`` (default) or ``'fp32'``:
                        with ``'fp16'`` the inner FP32 apply reads FP16 copies of
                        the factor blocks (one FP32 scale per row), half the bytes
@@ -125,6 +128,7 @@ def _opts(parameters):
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
         'Learned Network': o.get('Learned Network'),
         'Apply Graph': bool(o.get('Apply Graph', True)),
+        'Step Graphs': bool(o.get('Step Graphs', True)),
         'Apply Storage': o.get('Apply Storage', 'fp16'),
         'Background Factor': bool(o.get('Background Factor', False)),
         'Factor GEMM': o.get('Factor GEMM', 'fp32'),
@@ -584,11 +588,17 @@ class NativeKrylov:
     """The same FGMRES as :class:`Krylov`, run inside the library (tf_fgmres):
     one call per solve, no interpreter in the Arnoldi loop."""
 
-    def __init__(self, n, restart, dtype='float64'):
+    def __init__(self, n, restart, dtype='float64', graphs=True):
         self.n, self.m, self.dtype = n, restart, dtype
         h = ctypes.c_void_p()
         _lib.call('tf_krylov_create', ctypes.byref(h), n, restart, 1 if dtype == 'float32' else 0)
         self._h = h
+        self.set_graphs(graphs)
+
+    def set_graphs(self, on):
+        """Replay each Arnoldi step as a captured graph (on) or launch it eagerly."""
+        self.graphs = bool(on)
+        _lib.call('tf_krylov_set_graph', self._h, 1 if on else 0)
 
     def solve(self, jac, val, gauge, prec, b, x, tol, maxit, x_is_zero=False):
         kind, handle, ncell, dof = prec.native()
@@ -747,6 +757,10 @@ class Solver:
         k = self._krylov.get(key)
         if k is None or k.m != restart:
             k = self._krylov[key] = (NativeKrylov if native else Krylov)(self.itf.n, restart, dtype)
+        if native:
+            graphs = _opts(self.itf.parameters)['Step Graphs']
+            if k.graphs != graphs:
+                k.set_graphs(graphs)
         return k
 
     def gauge(self):
