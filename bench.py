@@ -47,6 +47,7 @@ METRIC = 'continuation steps/s at 1e-8 residual: learned+mixed vs FP64 GPU; asse
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0   # dense
 
 
 def log(*a):
@@ -280,7 +281,7 @@ def assembly_roofline(tf, torch, itf, x, reps=20):
 PROFILED = {
     5: ('k_gemm_mfma', 'mfma', 'flop'),
     3: ('k_panel', 'mfma', 'flop'),
-    9: ('line-factor apply (k_fwd, k_bwd_t, k_bwd_x, k_copy)', 'hbm', 'byte'),
+    9: ('line-factor apply (k_fwd, k_bwd_fused, k_bwd_t, k_bwd_x, k_copy)', 'hbm', 'byte'),
 }
 
 
@@ -420,6 +421,37 @@ def kernel_sample(n, reps=10):
     log('kernel sample done: %d applies, %d-float copy' % (reps, big))
 
 
+def conv_sample(reps=5, n=64):
+    """Short launch sequence for the learned network's MFMA kernels: the 3D
+    cavity's network (4 -> 32 -> 32 -> 4 channels, 3x3x3) on an n^3 grid,
+    ``reps`` forward passes then ``reps`` backward passes (weight and data
+    gradients), event-timed; 69,120 flop per cell per forward (2 x 27 taps x
+    (4*32 + 32*32 + 32*4)), the backward twice that."""
+    import torch
+    from transiflow_amd.learned import ConvNet
+    shape = (n, n, n)
+    ncell = n ** 3
+    net = ConvNet(3, 4, hidden=32, seed=0)
+    x = torch.rand(ncell * 4, dtype=torch.float32, device='cuda') - 0.5
+    gy = torch.rand(ncell * 4, dtype=torch.float32, device='cuda') - 0.5
+    net.forward(x, shape)
+    net.backward(gy, shape)
+    torch.cuda.synchronize()
+    out = {}
+    for name, fn, flop in (('forward', lambda: net.forward(x, shape), 69120.0 * ncell),
+                           ('backward', lambda: net.backward(gy, shape), 2 * 69120.0 * ncell)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        sec = e0.elapsed_time(e1) / reps * 1e-3
+        out[name] = {'us': sec * 1e6, 'TFLOPs': flop / sec / 1e12, 'frac_bf16_peak': flop / sec / 1e12 / BF16_PEAK_TFLOPS}
+    log('conv sample (%d^3, %d cells): %s' % (n, ncell, json.dumps(out)))
+    print(json.dumps({'conv_sample': out, 'cells': ncell}), flush=True)
+
+
 def batch_main(args):
     """64 cavity instances (square grids drawn from 32..256 step 32, target Re
     uniform in [100, 1500], numpy default_rng(2)), each Newton at Re = 0 then
@@ -475,6 +507,8 @@ def main():
     ap.add_argument('--workers', type=int, default=8, help='batch workload: concurrent worker streams')
     ap.add_argument('--kernel-sample', action='store_true',
                     help='run only the short launch sequence used for hardware-counter profiles')
+    ap.add_argument('--conv-sample', action='store_true',
+                    help="run only the learned network's forward and backward passes at 64^3 (MFMA profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the continuation hot path runs on a single MI355X '
@@ -483,6 +517,9 @@ def main():
 
     if args.kernel_sample:
         kernel_sample(args.n)
+        return
+    if args.conv_sample:
+        conv_sample()
         return
     if args.workload == 'batch':
         batch_main(args)

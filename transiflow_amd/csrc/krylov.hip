@@ -19,11 +19,20 @@
 namespace {
 
 constexpr int MD_THREADS = 256;
-constexpr int MD_BLOCKS = 256;   // fixed partition -> deterministic sums
+constexpr int MD_MAX_BLOCKS = 1024;
 constexpr int MAX_BASIS = 256;
 
 constexpr int MD_WAVES = MD_THREADS / 64;
 constexpr int MD_KB = 8;   // basis vectors per pass over w
+
+// 16-byte vectors: each lane moves 16 bytes per load, so a wavefront keeps
+// 1 KiB per basis vector in flight (streaming from HBM needs tens of KiB in
+// flight per CU).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = f32x4; static constexpr int N = 4; };
+template <> struct Vec16<double> { using type = f64x2; static constexpr int N = 2; };
 
 template <typename A>
 __device__ __forceinline__ A wave_sum(A v) {
@@ -31,18 +40,33 @@ __device__ __forceinline__ A wave_sum(A v) {
   return v;
 }
 
-template <typename T>
+// The block count of a multidot over n entries: a fixed function of n (so the
+// partition, and with it every sum, is deterministic), about two 16-byte
+// loads per lane and basis vector, at most MD_MAX_BLOCKS.
+int md_blocks(long n) {
+  const long per = (long)MD_THREADS * 2 * 4;
+  const long b = (n + per - 1) / per;
+  return (int)(b < 64 ? 64 : (b > MD_MAX_BLOCKS ? MD_MAX_BLOCKS : b));
+}
+
+// Block b's sums of V[i] . w into partial[i * nb + b].  Accumulation in the
+// vector type (FP32 vectors: FP32 accumulation inside each block), MD_KB
+// basis vectors per pass so w is read once per MD_KB vectors, reduced by
+// wavefront shuffles and then across the block's waves in a fixed order.
+// VEC: 16-byte loads over entries [b0, b1) counted in vectors (rows of V and w
+// 16-byte aligned, n a multiple of the vector width); else one entry per lane.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, const T* __restrict__ V, long ldv,
                                                                  const T* __restrict__ w,
                                                                  double* __restrict__ partial) {
-  // partial[i * MD_BLOCKS + b] = sum over block b's slice of V[i] . w, accumulated
-  // in the vector type (FP32 vectors: FP32 accumulation inside each block).
-  // MD_KB basis vectors per pass, so w is read once per MD_KB vectors; sums are
-  // reduced by wavefront shuffles, then across the block's waves in a fixed order.
   using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = VEC ? Vec16<T>::N : 1;
   __shared__ Acc sh[MD_WAVES][MD_KB];
-  const int chunk = (n + MD_BLOCKS - 1) / MD_BLOCKS;
-  const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+  const int nb = gridDim.x;
+  const int units = n / VN;
+  const int chunk = (units + nb - 1) / nb;
+  const int b0 = blockIdx.x * chunk, b1 = min(units, b0 + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i0 = 0; i0 < k; i0 += MD_KB) {
     const int kb = min(MD_KB, k - i0);
@@ -50,10 +74,24 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, c
 #pragma unroll
     for (int q = 0; q < MD_KB; ++q) acc[q] = 0;
     for (int e = b0 + threadIdx.x; e < b1; e += MD_THREADS) {
-      const Acc we = (Acc)w[e];
+      if constexpr (VEC) {
+        const VT we = reinterpret_cast<const VT*>(w)[e];
+        VT vq[MD_KB];
 #pragma unroll
-      for (int q = 0; q < MD_KB; ++q)
-        if (q < kb) acc[q] += (Acc)V[(long)(i0 + q) * ldv + e] * we;
+        for (int q = 0; q < MD_KB; ++q)
+          if (q < kb) vq[q] = reinterpret_cast<const VT*>(V + (long)(i0 + q) * ldv)[e];
+#pragma unroll
+        for (int q = 0; q < MD_KB; ++q)
+          if (q < kb) {
+#pragma unroll
+            for (int l = 0; l < VN; ++l) acc[q] += (Acc)vq[q][l] * (Acc)we[l];
+          }
+      } else {
+        const Acc we = (Acc)w[e];
+#pragma unroll
+        for (int q = 0; q < MD_KB; ++q)
+          if (q < kb) acc[q] += (Acc)V[(long)(i0 + q) * ldv + e] * we;
+      }
     }
 #pragma unroll
     for (int q = 0; q < MD_KB; ++q) {
@@ -64,19 +102,19 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, c
     if (threadIdx.x < kb) {
       Acc s = 0;
       for (int v = 0; v < MD_WAVES; ++v) s += sh[v][threadIdx.x];
-      partial[(long)(i0 + threadIdx.x) * MD_BLOCKS + blockIdx.x] = (double)s;
+      partial[(long)(i0 + threadIdx.x) * nb + blockIdx.x] = (double)s;
     }
     __syncthreads();
   }
 }
 
 // h[i] (+)= sum_b partial[i][b]; one workgroup per i, wavefront reductions.
-__global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, const double* __restrict__ partial,
+__global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, int nb, const double* __restrict__ partial,
                                                                double* __restrict__ h, int accumulate) {
   __shared__ double sh[MD_WAVES];
   const int i = blockIdx.x;
   double acc = 0.0;
-  for (int b = threadIdx.x; b < MD_BLOCKS; b += MD_THREADS) acc += partial[(long)i * MD_BLOCKS + b];
+  for (int b = threadIdx.x; b < nb; b += MD_THREADS) acc += partial[(long)i * nb + b];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -87,35 +125,76 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, const doub
   }
 }
 
-// w -= sum_i coef[i] V[i]   (coef on the device, FP64)
-template <typename T>
+// w += sign * sum_i coef[i] V[i]   (coef on the device, FP64; accumulation in
+// FP64 in the order i = 0, 1, ...).  VEC: 16-byte loads, four basis vectors'
+// loads issued together.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __restrict__ V, long ldv,
                                                    const double* __restrict__ coef, T* __restrict__ w, double sign) {
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = VEC ? Vec16<T>::N : 1;
   __shared__ double c[MAX_BASIS];
-  for (int i = threadIdx.x; i < k; i += blockDim.x) c[i] = coef[i];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) c[i] = sign * coef[i];
   __syncthreads();
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    double acc = (double)w[e];
-    for (int i = 0; i < k; ++i) acc += sign * c[i] * (double)V[(long)i * ldv + e];
-    w[e] = (T)acc;
+  const int units = n / VN;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < units; e += gridDim.x * blockDim.x) {
+    if constexpr (VEC) {
+      VT wv = reinterpret_cast<const VT*>(w)[e];
+      double acc[VN];
+#pragma unroll
+      for (int l = 0; l < VN; ++l) acc[l] = (double)wv[l];
+      int i = 0;
+      for (; i + 4 <= k; i += 4) {
+        VT a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = reinterpret_cast<const VT*>(V + (long)(i + q) * ldv)[e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int l = 0; l < VN; ++l) acc[l] += c[i + q] * (double)a[q][l];
+      }
+      for (; i < k; ++i) {
+        const VT a = reinterpret_cast<const VT*>(V + (long)i * ldv)[e];
+#pragma unroll
+        for (int l = 0; l < VN; ++l) acc[l] += c[i] * (double)a[l];
+      }
+#pragma unroll
+      for (int l = 0; l < VN; ++l) wv[l] = (T)acc[l];
+      reinterpret_cast<VT*>(w)[e] = wv;
+    } else {
+      double acc = (double)w[e];
+      for (int i = 0; i < k; ++i) acc += c[i] * (double)V[(long)i * ldv + e];
+      w[e] = (T)acc;
+    }
   }
 }
 
 // out = w / nrm where nrm is a device scalar (the next Arnoldi vector).
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void k_scale_by_inv(int n, const T* __restrict__ w, const double* __restrict__ nrm,
                                                       T* __restrict__ out) {
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = VEC ? Vec16<T>::N : 1;
   const double inv = 1.0 / *nrm;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-    out[e] = (T)((double)w[e] * inv);
+  const int units = n / VN;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < units; e += gridDim.x * blockDim.x) {
+    if constexpr (VEC) {
+      VT v = reinterpret_cast<const VT*>(w)[e];
+#pragma unroll
+      for (int l = 0; l < VN; ++l) v[l] = (T)((double)v[l] * inv);
+      reinterpret_cast<VT*>(out)[e] = v;
+    } else {
+      out[e] = (T)((double)w[e] * inv);
+    }
+  }
 }
 
-// sqrt(sum partial) into out
-__global__ __launch_bounds__(MD_THREADS) void k_norm_final(const double* __restrict__ partial,
+// sqrt(sum partial[0..nb)) into out
+__global__ __launch_bounds__(MD_THREADS) void k_norm_final(int nb, const double* __restrict__ partial,
                                                            double* __restrict__ out) {
   __shared__ double sh[MD_WAVES];
   double acc = 0.0;
-  for (int b = threadIdx.x; b < MD_BLOCKS; b += MD_THREADS) acc += partial[b];
+  for (int b = threadIdx.x; b < nb; b += MD_THREADS) acc += partial[b];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -124,6 +203,14 @@ __global__ __launch_bounds__(MD_THREADS) void k_norm_final(const double* __restr
     for (int v = 0; v < MD_WAVES; ++v) s += sh[v];
     *out = sqrt(s);
   }
+}
+
+// 16-byte access is possible when n and the leading dimension are multiples of
+// the vector width and the pointers are 16-byte aligned.
+template <typename T>
+bool vec_ok(int n, long ldv, const void* a, const void* b) {
+  constexpr int VN = Vec16<T>::N;
+  return n % VN == 0 && ldv % VN == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)b % 16) == 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -243,9 +330,43 @@ template <typename T>
 int multidot(int n, int k, const T* V, long ldv, const T* w, double* work, double* h, int accumulate,
              hipStream_t st) {
   if (k < 1 || k > MAX_BASIS) return tf::fail("multidot: bad basis size");
-  hipLaunchKernelGGL(k_multidot_partial<T>, dim3(MD_BLOCKS), dim3(MD_THREADS), 0, st, n, k, V, ldv, w, work);
-  hipLaunchKernelGGL(k_multidot_final, dim3(k), dim3(MD_THREADS), 0, st, k, work, h, accumulate);
+  const int nb = md_blocks(n);
+  if (vec_ok<T>(n, ldv, V, w))
+    hipLaunchKernelGGL((k_multidot_partial<T, true>), dim3(nb), dim3(MD_THREADS), 0, st, n, k, V, ldv, w, work);
+  else
+    hipLaunchKernelGGL((k_multidot_partial<T, false>), dim3(nb), dim3(MD_THREADS), 0, st, n, k, V, ldv, w, work);
+  hipLaunchKernelGGL(k_multidot_final, dim3(k), dim3(MD_THREADS), 0, st, k, nb, work, h, accumulate);
   TF_LAUNCH_CHECK("multidot");
+  return 0;
+}
+
+template <typename T>
+int multiaxpy(int n, int k, const T* V, long ldv, const double* coef, T* w, double sign, hipStream_t st) {
+  if (k < 0 || k > MAX_BASIS) return tf::fail("multiaxpy: bad basis size");
+  if (k == 0) return 0;
+  if (vec_ok<T>(n, ldv, V, w))
+    hipLaunchKernelGGL((k_multiaxpy<T, true>), dim3(grid_for(n / Vec16<T>::N)), dim3(256), 0, st, n, k, V, ldv,
+                       coef, w, sign);
+  else
+    hipLaunchKernelGGL((k_multiaxpy<T, false>), dim3(grid_for(n)), dim3(256), 0, st, n, k, V, ldv, coef, w, sign);
+  TF_LAUNCH_CHECK("multiaxpy");
+  return 0;
+}
+
+template <typename T>
+int normalize(int n, const T* w, double* work, double* nrm_dev, T* out, hipStream_t st) {
+  const int nb = md_blocks(n);
+  if (vec_ok<T>(n, n, w, w))
+    hipLaunchKernelGGL((k_multidot_partial<T, true>), dim3(nb), dim3(MD_THREADS), 0, st, n, 1, w, (long)n, w, work);
+  else
+    hipLaunchKernelGGL((k_multidot_partial<T, false>), dim3(nb), dim3(MD_THREADS), 0, st, n, 1, w, (long)n, w, work);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(MD_THREADS), 0, st, nb, work, nrm_dev);
+  if (vec_ok<T>(n, n, w, out))
+    hipLaunchKernelGGL((k_scale_by_inv<T, true>), dim3(grid_for(n / Vec16<T>::N)), dim3(256), 0, st, n, w, nrm_dev,
+                       out);
+  else
+    hipLaunchKernelGGL((k_scale_by_inv<T, false>), dim3(grid_for(n)), dim3(256), 0, st, n, w, nrm_dev, out);
+  TF_LAUNCH_CHECK("normalize");
   return 0;
 }
 
@@ -253,7 +374,7 @@ int multidot(int n, int k, const T* V, long ldv, const T* w, double* work, doubl
 
 extern "C" {
 
-int tf_krylov_work_size(void) { return MD_BLOCKS * MAX_BASIS; }
+int tf_krylov_work_size(void) { return MD_MAX_BLOCKS * MAX_BASIS; }
 int tf_krylov_max_basis(void) { return MAX_BASIS; }
 
 // h[0..k) (+)= V[0..k)^T w     (V row-major with leading dimension ldv)
@@ -269,39 +390,19 @@ int tf_multidot_f32(int n, int k, const float* V, long ldv, const float* w, doub
 // w += sign * sum_i coef[i] V[i]
 int tf_multiaxpy_f64(int n, int k, const double* V, long ldv, const double* coef, double* w, double sign,
                      void* stream) {
-  if (k < 0 || k > MAX_BASIS) return tf::fail("multiaxpy: bad basis size");
-  if (k == 0) return 0;
-  hipLaunchKernelGGL(k_multiaxpy<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, k, V, ldv, coef,
-                     w, sign);
-  TF_LAUNCH_CHECK("multiaxpy");
-  return 0;
+  return multiaxpy<double>(n, k, V, ldv, coef, w, sign, (hipStream_t)stream);
 }
 int tf_multiaxpy_f32(int n, int k, const float* V, long ldv, const double* coef, float* w, double sign,
                      void* stream) {
-  if (k < 0 || k > MAX_BASIS) return tf::fail("multiaxpy: bad basis size");
-  if (k == 0) return 0;
-  hipLaunchKernelGGL(k_multiaxpy<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, n, k, V, ldv, coef,
-                     w, sign);
-  TF_LAUNCH_CHECK("multiaxpy");
-  return 0;
+  return multiaxpy<float>(n, k, V, ldv, coef, w, sign, (hipStream_t)stream);
 }
 
 // nrm_dev = ||w||_2 (deterministic), then out = w / nrm_dev
 int tf_normalize_f64(int n, const double* w, double* work, double* nrm_dev, double* out, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_multidot_partial<double>, dim3(MD_BLOCKS), dim3(MD_THREADS), 0, st, n, 1, w, (long)n, w, work);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(MD_THREADS), 0, st, work, nrm_dev);
-  hipLaunchKernelGGL(k_scale_by_inv<double>, dim3(grid_for(n)), dim3(256), 0, st, n, w, nrm_dev, out);
-  TF_LAUNCH_CHECK("normalize");
-  return 0;
+  return normalize<double>(n, w, work, nrm_dev, out, (hipStream_t)stream);
 }
 int tf_normalize_f32(int n, const float* w, double* work, double* nrm_dev, float* out, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_multidot_partial<float>, dim3(MD_BLOCKS), dim3(MD_THREADS), 0, st, n, 1, w, (long)n, w, work);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(MD_THREADS), 0, st, work, nrm_dev);
-  hipLaunchKernelGGL(k_scale_by_inv<float>, dim3(grid_for(n)), dim3(256), 0, st, n, w, nrm_dev, out);
-  TF_LAUNCH_CHECK("normalize");
-  return 0;
+  return normalize<float>(n, w, work, nrm_dev, out, (hipStream_t)stream);
 }
 
 int tf_cellblock_setup(int ncell, int dof, const int* rp, const int* ci, const double* v, int gauge, double* inv,

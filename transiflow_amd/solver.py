@@ -53,7 +53,9 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
                        preconditioner is refactored (8, for 'Inner Tolerance';
-                       scaled by the reduction a step asks for); a refinement step that
+                       scaled by the reduction a step asks for; at least 1.5 times
+                       the iterations the preconditioner needed on the Jacobian it
+                       was made from); a refinement step that
                        stagnates on a lagged factor also refactors;
 ``'Factor GEMM'``      (mixed, line factor) ``'fp32'`` (default), ``'bf16x3'`` or
                        ``'bf16'``: the level GEMMs of the FP32 factor on FP32
@@ -1058,8 +1060,13 @@ class Solver:
             _lib.call('tf_axpby_f64', n, 1.0, ptr(x), 1.0, ptr(d), ptr(x), st)
             outer += 1
             # the limit is for 'Inner Tolerance': a tighter last step may take
-            # proportionally more iterations without the factor being staler
-            limit = opts['Reuse Limit'] * math.log(itol) / math.log(opts['Inner Tolerance'])
+            # proportionally more iterations without the factor being staler.  A
+            # preconditioner that is not exact even when fresh (the two-grid method)
+            # is judged against 1.5 times what it needed on its own Jacobian.
+            if jac.prec_exact and outer == 1 and itol == opts['Inner Tolerance']:
+                prec.fresh_inner_its = its
+            base = max(opts['Reuse Limit'], 1.5 * getattr(prec, 'fresh_inner_its', 0))
+            limit = base * math.log(itol) / math.log(opts['Inner Tolerance'])
             if not jac.prec_exact and its > limit:
                 # lagged factor too stale: refactor this Jacobian and keep refining
                 # (in the background: with the lagged factor until the new one is ready)
