@@ -19,22 +19,23 @@ def _cuda(a, dt):
     return torch.from_numpy(numpy.ascontiguousarray(a)).to(dt).cuda()
 
 
-def _csr(n, seed):
+def _csr(n, seed, per_row=9.0):
     rng = numpy.random.default_rng(seed)
-    A = sparse.random(n, n, density=min(1.0, 9.0 / n), random_state=rng, format='csr')
+    A = sparse.random(n, n, density=min(1.0, per_row / n), random_state=rng, format='csr')
     A = (A + sparse.eye(n, format='csr')).tocsr()
     A.sort_indices()
     return A
 
 
-@pytest.mark.parametrize('n', [5, 3000, 196608 + 3])
+@pytest.mark.parametrize('n,per_row', [(5, 9), (3000, 9), (196608 + 3, 9), (3000, 45)])
 @pytest.mark.parametrize('single', [False, True])
-def test_spmv_gauge_matches_scipy(hip, n, single):
-    """y = A x with column g left out of every row and y[g] = -x[g]."""
+def test_spmv_gauge_matches_scipy(hip, n, per_row, single):
+    """y = A x with column g left out of every row and y[g] = -x[g]; 45 entries
+    per row take the path for workgroups whose entries exceed the LDS stage."""
     import torch
     from transiflow_amd import _lib
     from transiflow_amd.vector import ptr, stream
-    A = _csr(n, 11)
+    A = _csr(n, 11, per_row)
     g = min(2, n - 1)
     x = numpy.random.default_rng(12).uniform(-1, 1, n)
     B = A.tolil()

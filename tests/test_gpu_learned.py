@@ -19,7 +19,8 @@ def _pad16(v):
 
 
 @pytest.mark.parametrize('shape,dim,cin,cout,relu', [((13, 9, 1), 2, 3, 32, 1), ((6, 5, 4), 3, 32, 4, 0),
-                                                     ((17, 7, 1), 2, 32, 32, 1), ((5, 4, 3), 3, 4, 32, 1)])
+                                                     ((17, 7, 1), 2, 32, 32, 1), ((5, 4, 3), 3, 4, 32, 1),
+                                                     ((20, 17, 9), 3, 32, 32, 1), ((19, 6, 7), 3, 4, 32, 1)])
 def test_conv_layer_parity(hip, shape, dim, cin, cout, relu):
     import torch
     from transiflow_amd import _lib

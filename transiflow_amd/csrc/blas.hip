@@ -244,7 +244,7 @@ template <typename TV, typename TX>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                               const TV* __restrict__ v, const TX* __restrict__ x, TX* __restrict__ y,
                                               int gauge) {
-  tf::spmv_rows<tf::SPMV_LANES>(n, rp, ci, v, x, y, gauge);
+  tf::spmv(n, rp, ci, v, x, y, gauge);
 }
 
 __global__ void k_set_entry(double* x, long i, double v) { x[i] = v; }
@@ -553,8 +553,8 @@ int tf_dot_f64(long n, const double* x, const double* y, double* work, double* o
 int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const double* x, double* y, int gauge,
                 void* stream) {
   TF_PROF_BEGIN(tf::PROF_SPMV, (hipStream_t)stream);
-  hipLaunchKernelGGL((k_spmv<double, double>), dim3(tf::spmv_blocks(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci,
-                     v, x, y, gauge);
+  hipLaunchKernelGGL((k_spmv<double, double>), dim3(tf::spmv_blocks<double>(n)), dim3(256), 0, (hipStream_t)stream, n, rp,
+                     ci, v, x, y, gauge);
   TF_PROF_END(tf::PROF_SPMV, (hipStream_t)stream);
   TF_LAUNCH_CHECK("k_spmv f64");
   return 0;
@@ -562,7 +562,7 @@ int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const doub
 
 int tf_spmv_f32(int n, const int* rp, const int* ci, const float* v, const float* x, float* y, int gauge,
                 void* stream) {
-  hipLaunchKernelGGL((k_spmv<float, float>), dim3(tf::spmv_blocks(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
+  hipLaunchKernelGGL((k_spmv<float, float>), dim3(tf::spmv_blocks<float>(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,
                      x, y, gauge);
   TF_LAUNCH_CHECK("k_spmv f32");
   return 0;

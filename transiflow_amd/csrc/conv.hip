@@ -110,52 +110,202 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, long ncell, int cin, 
   }
 }
 
-// Partial weight gradients over a chunk of cells per block:
-//   dWp[block][k][co] = sum_{cell in chunk} patch[cell][k] g[cell][co]
-// MFMA with A = patch^T (k x cells), B = g (cells x co).  One wave per
-// (k-tile, n-tile) pair, waves stride over the pairs.
-constexpr int WG_CHUNK = 256;
+// The same layer when cin is a multiple of 4 (every layer of the 3D network and
+// its data gradients): a lane's four K entries of one MFMA step are four
+// consecutive channels of one tap, so the A fragment is one 8-byte load of the
+// neighbour's activations; W is staged transposed ([n][k]) so the B fragment
+// is one 8-byte LDS read.  Each wave computes CF_MT tiles of 16 cells, reusing
+// each B fragment CF_MT times, and a block stages W once for 64 CF_MT cells.
+// The K order of the accumulation is that of k_conv_fwd.
+constexpr int CF_MT = 4;
 
-__global__ __launch_bounds__(256) void k_conv_wgrad(Grid3 g, long ncell, int cin, int cout, int kpad, int npad,
-                                                    const uint16_t* __restrict__ in, const uint16_t* __restrict__ gout,
-                                                    float* __restrict__ partial) {
-  const int K = (g.dim == 3 ? 27 : 9) * cin;
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_fwd_v(Grid3 g, long ncell, int cin, int cout, int kpad, int npad,
+                                                    const uint16_t* __restrict__ in, const uint16_t* __restrict__ W,
+                                                    const float* __restrict__ bias, int relu,
+                                                    float* __restrict__ out32, uint16_t* __restrict__ out16) {
+  extern __shared__ __align__(16) uint16_t Ws[];   // kpad * npad entries (dynamic: small layers keep occupancy)
+  for (int e = threadIdx.x; e < kpad * npad; e += 256) {
+    const int k = e / npad, n = e - k * npad;
+    Ws[n * kpad + k] = W[e];
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int q = lane >> 4, r = lane & 15;
-  const long c0 = (long)blockIdx.x * WG_CHUNK;
-  const int ktiles = kpad / 16, ntiles = npad / 16;
-  float* P = partial + (long)blockIdx.x * kpad * npad;
-  for (int tile = wave; tile < ktiles * ntiles; tile += 4) {
-    const int kt = tile / ntiles, nt = tile % ntiles;
-    const int k = kt * 16 + r;  // this lane's A row
-    int tap = 0, ci = 0;
-    if (k < K) {
-      tap = k / cin;
-      ci = k - tap * cin;
-    }
-    const int co = nt * 16 + r;  // this lane's B column
-    f32x4 acc = {0, 0, 0, 0};
-    for (int cc = 0; cc < WG_CHUNK; cc += 16) {
-      bf16x4 a, b;
+  const int K = (g.dim == 3 ? 27 : 9) * cin;
+  const long base = ((long)blockIdx.x * 4 + wave) * (CF_MT * 16);
+  int xm[CF_MT], ym[CF_MT], zm[CF_MT];
+  bool livem[CF_MT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long cell = c0 + cc + 4 * q + i;
-        uint16_t av = 0, bv = 0;
-        if (cell < ncell) {
-          if (k < K) {
-            const int x = (int)(cell % g.nx), y = (int)((cell / g.nx) % g.ny), z = (int)(cell / ((long)g.nx * g.ny));
-            const long nb = neighbour(g, x, y, z, tap);
-            if (nb >= 0) av = in[nb * cin + ci];
-          }
-          if (co < cout) bv = gout[cell * cout + co];
-        }
-        a[i] = (short)av;
-        b[i] = (short)bv;
+  for (int m = 0; m < CF_MT; ++m) {
+    const long cell = base + m * 16 + r;
+    livem[m] = cell < ncell;
+    const long c = livem[m] ? cell : 0;
+    xm[m] = (int)(c % g.nx);
+    ym[m] = (int)((c / g.nx) % g.ny);
+    zm[m] = (int)(c / ((long)g.nx * g.ny));
+  }
+  f32x4 acc[CF_MT][NT];
+#pragma unroll
+  for (int m = 0; m < CF_MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < kpad; k0 += 16) {
+    const int k = k0 + 4 * q;  // this lane's four K entries: channels ci..ci+3 of one tap
+    const bool kin = k < K;
+    const int tap = kin ? k / cin : 0;
+    const int ci = k - tap * cin;
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = g.dim == 3 ? tap / 9 - 1 : 0;
+    bf16x4 b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const bf16x4*>(&Ws[(t * 16 + r) * kpad + k]);
+    bf16x4 a[CF_MT];
+#pragma unroll
+    for (int m = 0; m < CF_MT; ++m) {
+      a[m] = bf16x4{0, 0, 0, 0};
+      const int X = xm[m] + dx, Y = ym[m] + dy, Z = zm[m] + dz;
+      if (kin && livem[m] && X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz) {
+        const long nb = (long)X + (long)g.nx * (Y + (long)g.ny * Z);
+        a[m] = *reinterpret_cast<const bf16x4*>(in + nb * cin + ci);
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) P[(long)(kt * 16 + 4 * q + v) * npad + nt * 16 + r] = acc[v];
+    for (int m = 0; m < CF_MT; ++m)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[t], acc[m][t], 0, 0, 0);
+  }
+  // D[4 q + v][r]: cell base + 16 m + 4 q + v, output channel 16 t + r
+#pragma unroll
+  for (int m = 0; m < CF_MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int co = t * 16 + r;
+      if (co >= cout) continue;
+      const float bco = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long c = base + m * 16 + 4 * q + v;
+        if (c >= ncell) continue;
+        float sv = acc[m][t][v] + bco;
+        if (relu) sv = sv > 0.f ? sv : 0.f;
+        if (out32) out32[c * cout + co] = sv;
+        if (out16) out16[c * cout + co] = f2bf(sv);
+      }
+    }
+}
+
+// Partial weight gradients, one block per WG_CHUNK cells:
+//   partial[block][k][co] = sum over the block's cells of patch[cell][k] g[cell][co]
+// as MFMAs with M = k, N = co and the cells as the reduction dimension.  The
+// block walks its cells WG_SUB at a time: the patch (all K values of WG_SUB
+// cells, zero outside the grid) and the gradient are staged transposed in LDS
+// ([k][cell], [co][cell]), so a lane's four reduction entries are one 8-byte
+// LDS read for A and for B.  The staging reads each (cell, tap) neighbour's
+// channels in 8- or 16-byte pieces.  Wave w of the WG_WAVES owns the output
+// tiles w, w + WG_WAVES, ... and keeps their accumulators in registers across
+// the whole chunk, so a block writes its partial once.
+constexpr int WG_CHUNK = 1024;
+constexpr int WG_SUB = 32;
+constexpr int WG_WAVES = 8;
+constexpr int WG_MAXT = 14;   // output tiles per wave: ceil(56 k-tiles x 2 n-tiles / 8)
+constexpr int WG_KMAX = 896;  // staged K rows
+
+template <int VEC>   // channels per staging load: 8 (16 bytes), 4 (8 bytes) or 1
+__global__ __launch_bounds__(64 * WG_WAVES) void k_conv_wgrad(Grid3 g, long ncell, int cin, int cout, int kpad,
+                                                              int npad, const uint16_t* __restrict__ in,
+                                                              const uint16_t* __restrict__ gout,
+                                                              float* __restrict__ partial) {
+  __shared__ __align__(16) uint16_t pt[WG_KMAX * WG_SUB];   // [kpad][WG_SUB]
+  __shared__ __align__(16) uint16_t gt[32 * WG_SUB];        // [npad][WG_SUB]
+  constexpr int NT = 64 * WG_WAVES;
+  const int taps = g.dim == 3 ? 27 : 9;
+  const int K = taps * cin;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = lane >> 4, r = lane & 15;
+  const int ktiles = kpad / 16, ntiles = npad / 16, ntot = ktiles * ntiles;
+  const int groups = cin / VEC;                 // channel pieces per (cell, tap)
+  const int staged_taps = (kpad + cin - 1) / cin;
+  const long c0 = (long)blockIdx.x * WG_CHUNK;
+  // K padding rows beyond the last staged tap stay zero for the whole chunk
+  for (int e = staged_taps * cin * WG_SUB + threadIdx.x; e < kpad * WG_SUB; e += NT) pt[e] = 0;
+  f32x4 acc[WG_MAXT];
+#pragma unroll
+  for (int t = 0; t < WG_MAXT; ++t) acc[t] = f32x4{0, 0, 0, 0};
+  for (int s0 = 0; s0 < WG_CHUNK && c0 + s0 < ncell; s0 += WG_SUB) {
+    // (piece, cell, tap) items: consecutive threads take consecutive pieces of one neighbour
+    for (int p = threadIdx.x; p < WG_SUB * staged_taps * groups; p += NT) {
+      const int gi = p % groups, rest = p / groups;
+      const int cl = rest % WG_SUB, tap = rest / WG_SUB;
+      const long cell = c0 + s0 + cl;
+      long nb = -1;
+      if (tap < taps && cell < ncell) {
+        const int ic = (int)cell;   // ncell < 2^31 (checked by the caller)
+        const int x = ic % g.nx, y = (ic / g.nx) % g.ny, z = ic / (g.nx * g.ny);
+        const int X = x + tap % 3 - 1, Y = y + (tap / 3) % 3 - 1, Z = z + (g.dim == 3 ? tap / 9 - 1 : 0);
+        if (X >= 0 && X < g.nx && Y >= 0 && Y < g.ny && Z >= 0 && Z < g.nz)
+          nb = (long)X + (long)g.nx * (Y + (long)g.ny * Z);
+      }
+      const int k0 = tap * cin + gi * VEC;
+      uint16_t v[VEC];
+      if (nb >= 0) {
+        const uint16_t* src = in + nb * cin + gi * VEC;
+        if constexpr (VEC == 8) {
+          const uint4 w = *reinterpret_cast<const uint4*>(src);
+          const unsigned u[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            v[2 * j] = (uint16_t)(u[j] & 0xffffu);
+            v[2 * j + 1] = (uint16_t)(u[j] >> 16);
+          }
+        } else if constexpr (VEC == 4) {
+          const uint2 w = *reinterpret_cast<const uint2*>(src);
+          v[0] = (uint16_t)(w.x & 0xffffu);
+          v[1] = (uint16_t)(w.x >> 16);
+          v[2] = (uint16_t)(w.y & 0xffffu);
+          v[3] = (uint16_t)(w.y >> 16);
+        } else {
+          v[0] = src[0];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) v[j] = 0;
+      }
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const int k = k0 + j;
+        if (k < kpad) pt[k * WG_SUB + cl] = k < K ? v[j] : (uint16_t)0;
+      }
+    }
+    for (int e = threadIdx.x; e < npad * WG_SUB; e += NT) {
+      const int co = e / WG_SUB, cl = e % WG_SUB;
+      const long cell = c0 + s0 + cl;
+      gt[co * WG_SUB + cl] = (co < cout && cell < ncell) ? gout[cell * cout + co] : (uint16_t)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < WG_MAXT; ++t) {
+      const int tile = wave + WG_WAVES * t;
+      if (tile < ntot) {
+        const int kt = tile / ntiles, nt = tile - kt * ntiles;
+#pragma unroll
+        for (int h = 0; h < WG_SUB; h += 16) {
+          const bf16x4 a = *reinterpret_cast<const bf16x4*>(&pt[(kt * 16 + r) * WG_SUB + h + 4 * q]);
+          const bf16x4 b = *reinterpret_cast<const bf16x4*>(&gt[(nt * 16 + r) * WG_SUB + h + 4 * q]);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* P = partial + (long)blockIdx.x * kpad * npad;
+#pragma unroll
+  for (int t = 0; t < WG_MAXT; ++t) {
+    const int tile = wave + WG_WAVES * t;
+    if (tile < ntot) {
+      const int kt = tile / ntiles, nt = tile - kt * ntiles;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) P[(long)(kt * 16 + 4 * q + v) * npad + nt * 16 + r] = acc[t][v];
+    }
   }
 }
 
@@ -169,14 +319,18 @@ __global__ __launch_bounds__(256) void k_sum_partials(int nb, long len, const fl
   out[e] = s;
 }
 
-// bias partials: bp[block][co] = sum over the block's chunk of g[cell][co] (FP32 gradient)
+// bias partials: bp[block][co] = sum over the block's WG_CHUNK cells of g[cell][co] (FP32 gradient)
 __global__ __launch_bounds__(256) void k_bias_partial(long ncell, int cout, const float* __restrict__ g,
                                                       float* __restrict__ bp) {
   __shared__ float sh[256];
   const long c0 = (long)blockIdx.x * WG_CHUNK;
   for (int co = 0; co < cout; ++co) {
-    const long cell = c0 + threadIdx.x;
-    sh[threadIdx.x] = cell < ncell ? g[cell * cout + co] : 0.f;
+    float acc = 0.f;
+    for (int cl = threadIdx.x; cl < WG_CHUNK; cl += 256) {
+      const long cell = c0 + cl;
+      if (cell < ncell) acc += g[cell * cout + co];
+    }
+    sh[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
       if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
@@ -280,9 +434,22 @@ int tf_conv_fwd(int nx, int ny, int nz, int dim, int cin, int cout, int kpad, in
   if (int e = check_layer(dim, cin, cout, kpad, npad)) return e;
   const long ncell = (long)nx * ny * nz;
   Grid3 g{nx, ny, nz, dim};
-  hipLaunchKernelGGL(k_conv_fwd, dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, (hipStream_t)stream, g, ncell,
-                     cin, cout, kpad, npad, (const uint16_t*)in, (const uint16_t*)W, bias, relu, out32,
-                     (uint16_t*)out16);
+  hipStream_t st = (hipStream_t)stream;
+  const uint16_t* in16 = (const uint16_t*)in;
+  const uint16_t* W16 = (const uint16_t*)W;
+  if (cin % 4 == 0 && ((uintptr_t)in % 8) == 0) {
+    const dim3 grid((unsigned)((ncell + 64 * CF_MT - 1) / (64 * CF_MT)));
+    const size_t lds = sizeof(uint16_t) * kpad * npad;
+    if (npad == 16)
+      hipLaunchKernelGGL(k_conv_fwd_v<1>, grid, dim3(256), lds, st, g, ncell, cin, cout, kpad, npad, in16, W16, bias,
+                         relu, out32, (uint16_t*)out16);
+    else
+      hipLaunchKernelGGL(k_conv_fwd_v<2>, grid, dim3(256), lds, st, g, ncell, cin, cout, kpad, npad, in16, W16, bias,
+                         relu, out32, (uint16_t*)out16);
+  } else {
+    hipLaunchKernelGGL(k_conv_fwd, dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, st, g, ncell, cin, cout, kpad,
+                       npad, in16, W16, bias, relu, out32, (uint16_t*)out16);
+  }
   TF_PROF_WORK(tf::PROF_CONV, 2.0 * ncell * (dim == 3 ? 27 : 9) * cin * cout);
   TF_LAUNCH_CHECK("k_conv_fwd");
   return 0;
@@ -301,11 +468,19 @@ int tf_conv_wgrad(int nx, int ny, int nz, int dim, int cin, int cout, int kpad, 
                   const void* gout16, const float* g32, float* work, float* dW, float* db, void* stream) {
   if (int e = check_layer(dim, cin, cout, kpad, npad)) return e;
   const long ncell = (long)nx * ny * nz;
+  if (kpad > WG_KMAX || ncell >= (1L << 31)) return tf::fail("conv wgrad: K or the grid too large");
   const long blocks = (ncell + WG_CHUNK - 1) / WG_CHUNK;
   hipStream_t st = (hipStream_t)stream;
   Grid3 g{nx, ny, nz, dim};
-  hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)blocks), dim3(256), 0, st, g, ncell, cin, cout, kpad, npad,
-                     (const uint16_t*)in, (const uint16_t*)gout16, work);
+  const dim3 wg_grid((unsigned)blocks), wg_block(64 * WG_WAVES);
+  const uint16_t* in16 = (const uint16_t*)in;
+  const uint16_t* g16 = (const uint16_t*)gout16;
+  if (cin % 8 == 0 && ((uintptr_t)in % 16) == 0)
+    hipLaunchKernelGGL(k_conv_wgrad<8>, wg_grid, wg_block, 0, st, g, ncell, cin, cout, kpad, npad, in16, g16, work);
+  else if (cin % 4 == 0 && ((uintptr_t)in % 8) == 0)
+    hipLaunchKernelGGL(k_conv_wgrad<4>, wg_grid, wg_block, 0, st, g, ncell, cin, cout, kpad, npad, in16, g16, work);
+  else
+    hipLaunchKernelGGL(k_conv_wgrad<1>, wg_grid, wg_block, 0, st, g, ncell, cin, cout, kpad, npad, in16, g16, work);
   const long len = (long)kpad * npad;
   hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, (int)blocks, len, work,
                      dW);

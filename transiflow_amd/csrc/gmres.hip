@@ -63,7 +63,7 @@ __global__ void k_set_desc(MatrixDesc* d, const int* rp, const int* ci, const vo
 template <typename T>
 __global__ __launch_bounds__(256) void k_spmv_desc(int n, const MatrixDesc* __restrict__ d, const T* __restrict__ x,
                                                    T* __restrict__ y) {
-  tf::spmv_rows<tf::SPMV_LANES>(n, d->rp, d->ci, (const T*)d->val, x, y, d->gauge);
+  tf::spmv(n, d->rp, d->ci, (const T*)d->val, x, y, d->gauge);
 }
 
 struct Ops {
@@ -125,10 +125,10 @@ int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
   const int n = k->n;
   Ops o{k, n, st};
   if (k->single)
-    hipLaunchKernelGGL(k_spmv_desc<float>, dim3(tf::spmv_blocks(n)), dim3(256), 0, st, n, k->desc,
+    hipLaunchKernelGGL(k_spmv_desc<float>, dim3(tf::spmv_blocks<float>(n)), dim3(256), 0, st, n, k->desc,
                        (const float*)o.Z(j), (float*)k->w);
   else
-    hipLaunchKernelGGL(k_spmv_desc<double>, dim3(tf::spmv_blocks(n)), dim3(256), 0, st, n, k->desc,
+    hipLaunchKernelGGL(k_spmv_desc<double>, dim3(tf::spmv_blocks<double>(n)), dim3(256), 0, st, n, k->desc,
                        (const double*)o.Z(j), (double*)k->w);
   TF_LAUNCH_CHECK("fgmres spmv");
   if (k->single) {
