@@ -127,7 +127,7 @@ DTYPES = {
 }
 
 
-LEARNED = {}   # the trained network of the learned16 variant (made once per process)
+LEARNED = {}   # dim -> the trained network of the learned16 variant (made once per process)
 CONVECTIVE = {}   # the steady convective state of the 'rbconv' workload (made once per process)
 
 
@@ -165,30 +165,35 @@ def convective_start(tf, torch, spec, max_steps=400):
     return xh
 
 
-def train_learned(tf, torch, n=48, target=1000.0):
+def train_learned(tf, torch, dim=2, n=None, target=1000.0, samples=None):
     """Train the learned correction as the A/B test does: triples (Jacobian,
     residual left by the FP32 line factor, FP64 correction) recorded from the
-    mixed continuation of an n x n cavity (Re 0 -> target), 400 Adam steps of
-    the BF16 MFMA network on 70% of them; the held-out 30% measure the gain."""
+    mixed continuation of a cavity (2D 48x48, or 3D 32^3 for the 3D workloads;
+    Re 0 -> target), 400 Adam steps of the BF16 MFMA network on 70% of them;
+    the held-out 30% measure the gain."""
     from transiflow_amd import learned
+    n = n or (48 if dim == 2 else 32)
+    samples = samples or (120 if dim == 2 else 60)
+    shape = (n, n, 1) if dim == 2 else (n, n, n)
     t0 = time.perf_counter()
 
     def run(itf):
         cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
         cont.continuation(cont.newton(itf.vector(), tol=1e-8), 'Reynolds Number', 0, target, 100)
 
-    itf, recs = learned.collect(lambda: tf.Interface({'HIP Solver': {'Precision': 'mixed'}}, n, n), run,
-                                every=2, max_samples=120)
+    itf, recs = learned.collect(lambda: tf.Interface({'HIP Solver': {'Precision': 'mixed'}}, *shape), run,
+                                every=2, max_samples=samples)
     pairs = learned.make_pairs(itf, recs)
     ntr = int(0.7 * len(pairs))
-    net = learned.ConvNet(2, 3, seed=0)
-    hist = learned.train(net, pairs[:ntr], (n, n, 1), steps=400, lr=1e-3)
+    net = learned.ConvNet(dim, dim + 1, seed=0)
+    hist = learned.train(net, pairs[:ntr], shape, steps=400, lr=1e-3)
     torch.cuda.synchronize()
-    info = {'grid': '%dx%d cavity, Re 0->%g' % (n, n, target), 'triples': len(pairs), 'train_steps': 400,
-            'loss_first20': float(sum(hist[:20]) / 20), 'loss_last20': float(sum(hist[-20:]) / 20),
+    info = {'grid': '%s cavity, Re 0->%g' % ('x'.join(str(v) for v in shape[:dim]), target), 'triples': len(pairs),
+            'train_steps': 400, 'loss_first20': float(sum(hist[:20]) / 20),
+            'loss_last20': float(sum(hist[-20:]) / 20),
             'heldout_gain_fraction_of_r1sq': learned.evaluate(net, itf, recs[ntr:]),
             'seconds': time.perf_counter() - t0}
-    LEARNED['net'], LEARNED['info'] = net, info
+    LEARNED[dim] = {'net': net, 'info': info}
     return net
 
 
@@ -198,7 +203,7 @@ def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     params['HIP Solver'] = dict(VARIANTS[variant], Restart=restart)
     params['HIP Solver'].update(spec.get('solver', {}))
     if variant == 'learned16':
-        params['HIP Solver']['Learned Network'] = LEARNED['net']
+        params['HIP Solver']['Learned Network'] = LEARNED[3 if spec['shape'][2] > 1 else 2]['net']
     itf = tf.Interface(params, *spec['shape'])
     cont = tf.Continuation(itf, newton_tolerance=1e-8, printer=None)
     start = itf.vector()
@@ -494,7 +499,7 @@ def main():
     ap.add_argument('--ds', type=float, default=100.0)
     ap.add_argument('--variants', default=None,
                     help="comma-separated solver variants (default: 'fp64,fp64reuse,mixed16,learned16' for the 2D "
-                         "cavity, 'fp64,mixed16' for the other workloads)")
+                         "cavity, 'fp64,mixed16,learned16' for the 3D cavities, 'fp64,mixed16' for the others)")
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-hbm-sizes', action='store_true',
                     help='skip the assembly and SpMV rates at 1024^2 and 64^3 (working sets beyond the cache)')
@@ -532,11 +537,13 @@ def main():
     maxit = args.steps if args.steps > 0 else 1000
     spec = workload(args.workload, args.n, args.target, args.ds)
     if args.variants is None:
-        args.variants = 'fp64,fp64reuse,mixed16,learned16' if args.workload == 'ldc' else 'fp64,mixed16'
+        args.variants = {'ldc': 'fp64,fp64reuse,mixed16,learned16', 'ldc3d64': 'fp64,mixed16,learned16',
+                         'ldc3d': 'fp64,mixed16,learned16'}.get(args.workload, 'fp64,mixed16')
     if 'learned16' in args.variants.split(','):
         log('training the learned correction (untimed set-up)')
-        train_learned(tf, torch)
-        log('  %s' % LEARNED['info'])
+        dim = 3 if spec['shape'][2] > 1 else 2
+        train_learned(tf, torch, dim=dim)
+        log('  %s' % LEARNED[dim]['info'])
     if args.warmup > 0:
         for var in args.variants.split(','):
             # also fills the GEMM tile tuner for this precision's shapes
@@ -639,7 +646,7 @@ def main():
                                            'worst_relative_residual', 'background_factorizations', 'learned',
                                            'preconditioner', 'device_memory_GB')}
                      for v, r in results.items()},
-        'learned_training': LEARNED.get('info'),
+        'learned_training': {d: v['info'] for d, v in LEARNED.items()} or None,
         'convective_start': CONVECTIVE.get('info'),
         'final_state_rel_err_vs_fp64': agree,
         'speedup_vs_fp64': {v: r['steps_per_s'] / base['steps_per_s'] for v, r in results.items()} if base else {},

@@ -226,3 +226,28 @@ def test_solve_rejects_bad_rhs_shapes():
     for bad in (numpy.zeros(11), numpy.zeros((11, 2)), numpy.zeros((12, 2, 1)), numpy.zeros(0)):
         with pytest.raises(ValueError):
             s.solve(None, bad)
+
+
+# -- preconditioner choice under 'auto' ------------------------------------------
+
+def _grid(nx, ny, nz, periodic=False):
+    from types import SimpleNamespace
+    dim = 3 if nz > 1 else 2
+    dof = dim + 1
+    d = SimpleNamespace(x_periodic=periodic, y_periodic=False, z_periodic=False)
+    return SimpleNamespace(nx=nx, ny=ny, nz=nz, dim=dim, dof=dof, n=nx * ny * nz * dof, discretization=d)
+
+
+def test_auto_preconditioner_choice():
+    """'auto' takes the exact line (plane) factor when it fits the memory budget,
+    the two-grid method for 3D grids beyond it, and otherwise raises a
+    SolverError that names the factor size, instead of falling back to a
+    preconditioner that does not converge."""
+    from transiflow_amd.solver import SolverError, auto_preconditioner
+    assert auto_preconditioner(_grid(256, 256, 1), False) == 'line'
+    assert auto_preconditioner(_grid(32, 32, 32), False) == 'line'
+    assert auto_preconditioner(_grid(64, 64, 64), False) == 'multigrid'
+    assert auto_preconditioner(_grid(64, 64, 64), True) == 'multigrid'
+    for g in (_grid(63, 63, 63), _grid(64, 64, 64, periodic=True), _grid(4096, 4096, 1)):
+        with pytest.raises(SolverError, match='no preconditioner for this grid.*GB'):
+            auto_preconditioner(g, False)
