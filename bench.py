@@ -104,6 +104,8 @@ VARIANTS = {
     'mixed': {'Precision': 'mixed', 'Apply Storage': 'fp32'},
     'mixed16': {'Precision': 'mixed', 'Apply Storage': 'fp16'},
     'mixed16eager': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Step Graphs': False},
+    'mixed16seq': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Pair Solves': False},
+    'fp64seq': {'Precision': 'fp64', 'Pair Solves': False},
     'mixed16bg': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Background Factor': True},
     'mixed16s3': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16x3'},
     'mixed16s1': {'Precision': 'mixed', 'Apply Storage': 'fp16', 'Factor GEMM': 'bf16'},
@@ -118,6 +120,9 @@ DTYPES = {
                'of the factor blocks with f32 row scales)',
     'mixed16eager': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES with eager Arnoldi launches, line factor; '
                     'apply reads f16 copies of the factor blocks with f32 row scales)',
+    'mixed16seq': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor; apply reads f16 copies of '
+                  'the factor blocks with f32 row scales; the two corrector solves one after the other)',
+    'fp64seq': 'f64 (the two corrector solves one after the other)',
     'mixed16bg': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor factored on a side stream; '
                  'apply reads f16 copies of the factor blocks with f32 row scales)',
     'mixed16s3': 'f64 (residual, refinement, Newton) + f32 (inner FGMRES, line factor with its level GEMMs on '
@@ -224,7 +229,7 @@ def run_continuation(tf, torch, spec, variant, maxit, restart=30):
     free1, _ = torch.cuda.mem_get_info()
     acc = sum(1 for h in cont.history if h[0] == 'accepted')
     iters = sum(h[1] for h in cont.history)
-    fact = sum(1 for entry in itf.solver.log if entry[3])
+    fact = sum(int(entry[3]) for entry in itf.solver.log)
     gm = sum(entry[0] for entry in itf.solver.log)
     worst = max(entry[1] for entry in itf.solver.log)
     bg = itf.solver._bg

@@ -165,6 +165,14 @@ int tf_krylov_set_graph(tf_krylov* k, int on);
 int tf_fgmres(tf_krylov* k, const int* row_ptr, const int* col, const void* val, int gauge, int prec_kind,
               void* prec, int ncell, int dof, const void* b, void* x, double tol, int maxit, int x_is_zero,
               int* its_out, double* rel_out, void* stream);
+/* Two systems with one matrix and preconditioner (A x0 = b0, A x1 = b1; the two solves
+ * with one Jacobian of a Newton-corrector iteration, Continuation.py:158-159), each
+ * with its own Krylov handle (same n, restart, precision) and tolerance, in lockstep:
+ * while both are in a restart cycle one tf_bcr_solve2_* serves both.  its_out,
+ * rel_out: two entries each. */
+int tf_fgmres2(tf_krylov* k0, tf_krylov* k1, const int* row_ptr, const int* col, const void* val, int gauge,
+               int prec_kind, void* prec, int ncell, int dof, const void* b0, const void* b1, void* x0, void* x1,
+               double tol0, double tol1, int maxit, int x_is_zero, int* its_out, double* rel_out, void* stream);
 
 /* Cell-block Jacobi preconditioner (dof x dof diagonal blocks of the gauged matrix). */
 int tf_cellblock_setup(int ncell, int dof, const int* row_ptr, const int* col, const double* val,
@@ -196,6 +204,11 @@ int tf_bcr_factor(tf_bcr* h, const int* row_ptr, const int* col, const double* v
 int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream);
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream);
 int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
+/* Two right-hand sides in one pass over the factor blocks: x0 = M r0, x1 = M r1, each
+ * bit-identical to tf_bcr_solve_*; for the two solves with one Jacobian of every
+ * Newton-corrector iteration (Continuation.py:158-159: F_x z1 = -F, F_x z2 = F_mu). */
+int tf_bcr_solve2_f64(tf_bcr* h, const double* r0, const double* r1, double* x0, double* x1, void* stream);
+int tf_bcr_solve2_f32(tf_bcr* h, const float* r0, const float* r1, float* x0, float* x1, void* stream);
 /* on != 0: the apply is captured once (per vector type) into a hipGraph and replayed with
  * one launch plus two staging copies (no reference counterpart; launch overhead on small
  * grids and batches). */

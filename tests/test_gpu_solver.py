@@ -344,6 +344,39 @@ def test_graph_apply_matches_eager(hip, shape):
         P.set_graph(False)
 
 
+@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1), (8, 8, 6)])
+@pytest.mark.parametrize('storage', ['fp64', 'fp32', 'fp16'])
+def test_pair_apply_matches_single(hip, shape, storage):
+    """Two right-hand sides through one pass over the factor blocks give, bit for
+    bit, the two single applies: x- and y-line orders (64 x 40 factors its
+    shorter y-lines) and 3D planes, FP64 / FP32 / FP16-copy blocks, eager and
+    replayed."""
+    import torch
+    itf = hip.Interface({'Reynolds Number': 400}, *shape)
+    n = itf.n
+    rng = numpy.random.default_rng(33)
+    J = itf.jacobian(itf.vector_from_array(rng.uniform(-1, 1, n) * 0.1))
+    single = storage != 'fp64'
+    P = itf.solver._build(J, 'line', single)
+    if single:
+        P.set_half(storage == 'fp16')
+    dt = torch.float32 if single else torch.float64
+    r0 = torch.from_numpy(rng.uniform(-1, 1, n)).to(dt).cuda()
+    r1 = torch.from_numpy(rng.uniform(-1, 1, n)).to(dt).cuda()
+    apply = P.apply32 if single else P.apply
+    for graph in (False, True, True):
+        P.set_graph(graph)
+        z0, z1 = torch.empty_like(r0), torch.empty_like(r0)
+        apply(r0, z0)
+        apply(r1, z1)
+        p0, p1 = torch.full_like(r0, numpy.nan), torch.full_like(r0, numpy.nan)
+        P.apply_pair(r0, r1, p0, p1, single=single)
+        torch.cuda.synchronize()
+        assert torch.isfinite(z0).all() and torch.isfinite(z1).all()
+        assert torch.equal(p0, z0) and torch.equal(p1, z1), (graph, storage)
+    P.set_graph(True)
+
+
 def test_csr_combination_matches_scipy(hip):
     """jacobian(x) - mass / (theta dt) on the device equals scipy's sparse
     arithmetic on the same matrices (TimeIntegration.py:56)."""
@@ -492,3 +525,30 @@ def test_shifted_complex_solve_matches_splu(hip, alpha, beta):
           flush=True)
     assert numpy.iscomplexobj(y) and y.shape == b.shape
     assert rel <= 1e-10 and err <= 1e-8
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1)])
+def test_solve_pair_matches_two_solves(hip, precision, shape):
+    """The two corrector solves in lockstep (solve_pair) meet the tolerance and
+    agree with two single solves of the same systems, also with a lagged
+    (mixed) factor of an earlier Jacobian; each is logged as a solve."""
+    itf = hip.Interface({'Reynolds Number': 300, 'HIP Solver': {'Precision': precision}}, *shape)
+    ref = hip.Interface({'Reynolds Number': 300, 'HIP Solver': {'Precision': precision, 'Pair Solves': False}},
+                        *shape)
+    rng = numpy.random.default_rng(35)
+    n = itf.n
+    for step in range(3):
+        state = rng.uniform(-1, 1, n) * 0.05 * (step + 1)
+        r0, r1 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        J, Jr = itf.jacobian(itf.vector_from_array(state)), ref.jacobian(ref.vector_from_array(state))
+        nlog = len(itf.solver.log)
+        y0, y1 = itf.solve_pair(J, itf.vector_from_array(r0), itf.vector_from_array(r1))
+        assert len(itf.solver.log) == nlog + 2
+        z0, z1 = ref.solve(Jr, ref.vector_from_array(r0)), ref.solve(Jr, ref.vector_from_array(r1))
+        for y, z, e in ((y0, z0, itf.solver.log[-2]), (y1, z1, itf.solver.log[-1])):
+            y, z = itf.array_from_vector(y), ref.array_from_vector(z)
+            err = numpy.linalg.norm(y - z) / numpy.linalg.norm(z)
+            print('%s %s step %d: its %d rel %.2e, vs single solve %.2e' % (shape, precision, step, e[0], e[1], err),
+                  flush=True)
+            assert e[1] <= 1e-10 and err <= 1e-7

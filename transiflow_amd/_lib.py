@@ -83,6 +83,8 @@ SIGNATURES = {
     'tf_krylov_set_graph': (_c_int, [_vp, _c_int]),
     'tf_fgmres': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_double, _c_int,
                            _c_int, _ip, _dp, _vp]),
+    'tf_fgmres2': (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                            _c_double, _c_double, _c_int, _c_int, _ip, _dp, _vp]),
     'tf_csr_add_count': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _vp]),
     'tf_csr_add_count_noalloc': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _ip, _vp]),
     'tf_csr_add_fill': (_c_int, [_c_int, _vp, _vp, _vp, _c_double, _vp, _vp, _vp, _c_double, _vp, _vp, _vp, _vp]),
@@ -94,6 +96,8 @@ SIGNATURES = {
     'tf_bcr_status': (_c_int, [_vp, _ip, _vp]),
     'tf_bcr_solve_f64': (_c_int, [_vp, _vp, _vp, _vp]),
     'tf_bcr_solve_f32': (_c_int, [_vp, _vp, _vp, _vp]),
+    'tf_bcr_solve2_f64': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    'tf_bcr_solve2_f32': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'tf_bcr_set_graph': (_c_int, [_vp, _c_int]),
     'tf_bcr_set_half': (_c_int, [_vp, _c_int, _vp]),
     'tf_bcr_set_gemm_split': (_c_int, [_vp, _c_int]),

@@ -44,7 +44,7 @@ def run_instance(inst, precision='mixed', ds=100.0, tol=1e-8):
     return {'x': xh, 'mu': mu, 'seconds': time.perf_counter() - t0,
             'accepted': sum(1 for h in cont.history if h[0] == 'accepted'),
             'iterations': sum(h[1] for h in cont.history) + cont.newton_iterations,
-            'factorizations': sum(1 for entry in itf.solver.log if entry[3])}
+            'factorizations': sum(int(entry[3]) for entry in itf.solver.log)}
 
 
 def run_batch(instances, workers=8, precision='mixed', ds=100.0, tol=1e-8):

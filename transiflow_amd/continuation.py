@@ -112,8 +112,15 @@ class Continuation:
                 dx, dmu = self.interface.solve(jac, -fval, -rnp1, dflval, 2 * self.zeta * diff,
                                                2 * (1 - self.zeta) * (mu - mu0))
             else:
-                z1 = self.interface.solve(jac, -fval)
-                z2 = self.interface.solve(jac, dflval)
+                # both right-hand sides are known before either solve: a backend with
+                # solve_pair solves them together (one pass over its factor per
+                # preconditioner application serves both)
+                pair = getattr(self.interface, 'solve_pair', None)
+                if pair is not None:
+                    z1, z2 = pair(jac, -fval, dflval)
+                else:
+                    z1 = self.interface.solve(jac, -fval)
+                    z2 = self.interface.solve(jac, dflval)
                 dmu = (-rnp1 - 2 * self.zeta * diff.dot(z1)) / (2 * (1 - self.zeta) * (mu - mu0)
                                                                 - 2 * self.zeta * diff.dot(z2))
                 dx = z1 - dmu * z2

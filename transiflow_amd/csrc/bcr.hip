@@ -268,56 +268,82 @@ struct Vec16<double> {
 __host__ __device__ __forceinline__ int rows_per_block(int G) { return 256 / G; }
 inline int group_for(int batch) { return batch >= 32 ? 16 : (batch >= 8 ? 32 : 64); }
 
-// sum over c of a[c] x[c] by the G-lane group g (the result is in every lane of the group)
-template <typename T, typename V>
-__device__ __forceinline__ double group_dot(const T* __restrict__ a, const V* __restrict__ x, int m, int g, int G,
-                                            bool vec) {
-  double acc = 0.0;
+// NC columns: sum over c of a[c] x_j[c] (x_j = x + j cs) by the G-lane group g
+// (every lane of the group gets the results).  The row is read once for all
+// columns; each column's sum has the order of the single-column case, so an NC
+// column apply equals NC single-column applies bit for bit.
+template <int NC, typename T, typename V>
+__device__ __forceinline__ void group_dot(const T* __restrict__ a, const V* __restrict__ x, long cs, int m, int g,
+                                          int G, bool vec, double (&out)[NC]) {
+  double acc[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
   if constexpr (sizeof(T) == sizeof(V)) {
     if (vec) {
       typedef typename Vec16<T>::type VT;
       constexpr int W = Vec16<T>::W;
       const VT* av = (const VT*)a;
-      const VT* xv = (const VT*)x;
 #pragma unroll 4
       for (int c = g; c < m / W; c += G) {
-        const VT p = av[c], q = xv[c];
+        const VT p = av[c];
 #pragma unroll
-        for (int w = 0; w < W; ++w) acc += (double)p[w] * (double)q[w];
+        for (int j = 0; j < NC; ++j) {
+          const VT q = ((const VT*)(x + j * cs))[c];
+#pragma unroll
+          for (int w = 0; w < W; ++w) acc[j] += (double)p[w] * (double)q[w];
+        }
       }
     } else {
-      for (int c = g; c < m; c += G) acc += (double)a[c] * (double)x[c];
+      for (int c = g; c < m; c += G)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j] += (double)a[c] * (double)x[j * cs + c];
     }
   } else {
-    for (int c = g; c < m; c += G) acc += (double)a[c] * (double)x[c];
+    for (int c = g; c < m; c += G)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += (double)a[c] * (double)x[j * cs + c];
   }
-  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
-  return acc;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, G);
+    out[j] = acc[j];
+  }
 }
 
-// FP16 row against an FP32 vector: 8 halves (16 bytes) and two float4 per lane per step
-__device__ __forceinline__ double group_dot(const f16* __restrict__ a, const float* __restrict__ x, int m, int g,
-                                            int G, bool vec) {
-  double acc = 0.0;
+// FP16 row against FP32 vectors: 8 halves (16 bytes) and two float4 per lane per step
+template <int NC>
+__device__ __forceinline__ void group_dot(const f16* __restrict__ a, const float* __restrict__ x, long cs, int m,
+                                          int g, int G, bool vec, double (&out)[NC]) {
+  double acc[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
   if (vec) {
     const f16x8* av = (const f16x8*)a;
-    const f32x4* xv = (const f32x4*)x;
 #pragma unroll 2
     for (int c = g; c < m / 8; c += G) {
       const f16x8 p = av[c];
-      const f32x4 q0 = xv[2 * c], q1 = xv[2 * c + 1];
-      float s = 0.0f;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) s = fmaf((float)p[w], q0[w], s);
+      for (int j = 0; j < NC; ++j) {
+        const f32x4* xv = (const f32x4*)(x + j * cs);
+        const f32x4 q0 = xv[2 * c], q1 = xv[2 * c + 1];
+        float sv = 0.0f;
 #pragma unroll
-      for (int w = 0; w < 4; ++w) s = fmaf((float)p[4 + w], q1[w], s);
-      acc += (double)s;
+        for (int w = 0; w < 4; ++w) sv = fmaf((float)p[w], q0[w], sv);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) sv = fmaf((float)p[4 + w], q1[w], sv);
+        acc[j] += (double)sv;
+      }
     }
   } else {
-    for (int c = g; c < m; c += G) acc += (double)(float)a[c] * (double)x[c];
+    for (int c = g; c < m; c += G)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += (double)(float)a[c] * (double)x[j * cs + c];
   }
-  for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
-  return acc;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, G);
+    out[j] = acc[j];
+  }
 }
 
 // Rows of a batch of m x m blocks in the factor's storage type.
@@ -336,37 +362,55 @@ struct HalfBlocks {
   __device__ __forceinline__ double scale(int b, int r) const { return (double)s[b][r]; }
 };
 
-template <typename B, typename V>
-__device__ __forceinline__ double row_dot(const B& blk, int b, int r, const V* __restrict__ x, int m, int g, int G,
-                                          bool vec) {
-  return blk.scale(b, r) * group_dot(blk.row(b, r, m), x, m, g, G, vec);
+template <int NC, typename B, typename V>
+__device__ __forceinline__ void row_dot(const B& blk, int b, int r, const V* __restrict__ x, long cs, int m, int g,
+                                        int G, bool vec, double (&out)[NC]) {
+  group_dot<NC>(blk.row(b, r, m), x, cs, m, g, G, vec, out);
+  const double sc = blk.scale(b, r);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) out[j] *= sc;
 }
 
+// Every vector argument below holds NC columns at stride cs.
+
 // rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines
-template <typename B, typename V>
+template <typename B, typename V, int NC>
 __global__ __launch_bounds__(256) void k_fwd(int m, int G, int vec, const int* __restrict__ k_line,
                                              const int* __restrict__ e1, const int* __restrict__ e2, B P1, B P2,
-                                             V* __restrict__ rw) {
+                                             V* __restrict__ rw, long cs) {
   const int b = blockIdx.y;
   const int g = threadIdx.x % G;
   const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
-  double acc = 0.0;
-  if (e1[b] >= 0) acc += row_dot(P1, b, rr, rw + (long)e1[b] * m, m, g, G, vec);
-  if (e2[b] >= 0) acc += row_dot(P2, b, rr, rw + (long)e2[b] * m, m, g, G, vec);
+  double acc[NC], d[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  if (e1[b] >= 0) {
+    row_dot<NC>(P1, b, rr, rw + (long)e1[b] * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+  }
+  if (e2[b] >= 0) {
+    row_dot<NC>(P2, b, rr, rw + (long)e2[b] * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+  }
   if (live && g == 0) {
-    V* out = rw + (long)k_line[b] * m + r;
-    *out = (V)((double)*out - acc);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      V* out = rw + j * cs + (long)k_line[b] * m + r;
+      *out = (V)((double)*out - acc[j]);
+    }
   }
 }
 
 // t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, levels >= 1)
-template <typename B, typename V>
+template <typename B, typename V, int NC>
 __global__ __launch_bounds__(256) void k_bwd_t(int m, int G, int vec, const int* __restrict__ e_line,
                                                const int* __restrict__ a_line, const int* __restrict__ b_line,
                                                B Lb, B Ub, const V* __restrict__ rw, const V* __restrict__ x,
-                                               V* __restrict__ tbuf) {
+                                               V* __restrict__ tbuf, long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
   const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
@@ -374,54 +418,75 @@ __global__ __launch_bounds__(256) void k_bwd_t(int m, int G, int vec, const int*
   const int rr = live ? r : 0;
   const long row = (long)e_line[bi] * m + rr;
   const int a = a_line[bi], bb = b_line[bi];
-  double acc = 0.0;
-  if (a >= 0) acc += row_dot(Lb, bi, rr, x + (long)a * m, m, g, G, vec);
-  if (bb >= 0) acc += row_dot(Ub, bi, rr, x + (long)bb * m, m, g, G, vec);
-  if (live && g == 0) tbuf[row] = (V)((double)rw[row] - acc);
+  double acc[NC], d[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  if (a >= 0) {
+    row_dot<NC>(Lb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+  }
+  if (bb >= 0) {
+    row_dot<NC>(Ub, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+  }
+  if (live && g == 0)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) tbuf[j * cs + row] = (V)((double)rw[j * cs + row] - acc[j]);
 }
 
 // Level 0: t[e] = rw[e] - L_e x[a] - U_e x[b] with the sparse couplings.  Thread per row.
-template <typename V>
+template <typename V, int NC>
 __global__ __launch_bounds__(256) void k_bwd_t_sparse(int m, const int* __restrict__ e_line,
                                                       const int* __restrict__ a_line, const int* __restrict__ b_line,
                                                       const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
                                                       const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
                                                       const V* __restrict__ rw, const V* __restrict__ x,
-                                                      V* __restrict__ tbuf) {
+                                                      V* __restrict__ tbuf, long cs) {
   const int bi = blockIdx.y;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
   const long row = (long)e_line[bi] * m + r;
   const int a = a_line[bi], bb = b_line[bi];
-  double acc = 0.0;
-  if (a >= 0)
-    for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)x[(long)a * m + Lr[row * KC + q].idx];
-  if (bb >= 0)
-    for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)x[(long)bb * m + Ur[row * KC + q].idx];
-  tbuf[row] = (V)((double)rw[row] - acc);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const V* xj = x + j * cs;
+    double acc = 0.0;
+    if (a >= 0)
+      for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)xj[(long)a * m + Lr[row * KC + q].idx];
+    if (bb >= 0)
+      for (int q = 0; q < nUr[row]; ++q)
+        acc += Ur[row * KC + q].val * (double)xj[(long)bb * m + Ur[row * KC + q].idx];
+    tbuf[j * cs + row] = (V)((double)rw[j * cs + row] - acc);
+  }
 }
 
 // x[e] = Y_e t[e]
-template <typename B, typename V>
+template <typename B, typename V, int NC>
 __global__ __launch_bounds__(256) void k_bwd_x(int m, int G, int vec, const int* __restrict__ e_line, B Yb,
-                                               const V* __restrict__ tbuf, V* __restrict__ x) {
+                                               const V* __restrict__ tbuf, V* __restrict__ x, long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
   const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
   const bool live = r < m;
   const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
-  const double acc = row_dot(Yb, bi, rr, tbuf + base, m, g, G, vec);
-  if (live && g == 0) x[base + r] = (V)acc;
+  double d[NC];
+  row_dot<NC>(Yb, bi, rr, tbuf + base, cs, m, g, G, vec, d);
+  if (live && g == 0)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j * cs + base + r] = (V)d[j];
 }
 
 // Levels >= 1 of a factor made with the products Y L and Y U in place of L and U
 // (tf_bcr_set_fused): x[e] = Y_e rw[e] - (Y_e L_e) x[a] - (Y_e U_e) x[b] in one
 // pass, the same three blocks read and no intermediate vector.
-template <typename B, typename V>
+template <typename B, typename V, int NC>
 __global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const int* __restrict__ e_line,
                                                    const int* __restrict__ a_line, const int* __restrict__ b_line,
-                                                   B Yb, B YLb, B YUb, const V* __restrict__ rw, V* __restrict__ x) {
+                                                   B Yb, B YLb, B YUb, const V* __restrict__ rw, V* __restrict__ x,
+                                                   long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
   const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
@@ -429,10 +494,21 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const 
   const int rr = live ? r : 0;
   const long base = (long)e_line[bi] * m;
   const int a = a_line[bi], bb = b_line[bi];
-  double acc = row_dot(Yb, bi, rr, rw + base, m, g, G, vec);
-  if (a >= 0) acc -= row_dot(YLb, bi, rr, x + (long)a * m, m, g, G, vec);
-  if (bb >= 0) acc -= row_dot(YUb, bi, rr, x + (long)bb * m, m, g, G, vec);
-  if (live && g == 0) x[base + r] = (V)acc;
+  double acc[NC], d[NC];
+  row_dot<NC>(Yb, bi, rr, rw + base, cs, m, g, G, vec, acc);
+  if (a >= 0) {
+    row_dot<NC>(YLb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+  }
+  if (bb >= 0) {
+    row_dot<NC>(YUb, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+  }
+  if (live && g == 0)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j * cs + base + r] = (V)acc[j];
 }
 
 // copy a batch of m x m blocks: dst[b] = src[b]   (grid.y = batch)
@@ -445,23 +521,27 @@ __global__ __launch_bounds__(256) void k_copy_blocks(long mm, T* const* __restri
 
 // Level 0, kept lines: rw[k] -= L_k y[e1] + U_k y[e2] with the sparse couplings and
 // y[e] = Y_e rw[e] (so P1 rw[e1] = L_k Y_e1 rw[e1] without reading P1).  Thread per row.
-template <typename V>
+template <typename V, int NC>
 __global__ __launch_bounds__(256) void k_fwd_sparse(int m, const int* __restrict__ k_line,
                                                     const int* __restrict__ e1, const int* __restrict__ e2,
                                                     const Cpl* __restrict__ Lr, const int* __restrict__ nLr,
                                                     const Cpl* __restrict__ Ur, const int* __restrict__ nUr,
-                                                    const V* __restrict__ y, V* __restrict__ rw) {
+                                                    const V* __restrict__ y, V* __restrict__ rw, long cs) {
   const int b = blockIdx.y;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
   const long row = (long)k_line[b] * m + r;
   const int a = e1[b], c = e2[b];
-  double acc = 0.0;
-  if (a >= 0)
-    for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)y[(long)a * m + Lr[row * KC + q].idx];
-  if (c >= 0)
-    for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)y[(long)c * m + Ur[row * KC + q].idx];
-  rw[row] = (V)((double)rw[row] - acc);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const V* yj = y + j * cs;
+    double acc = 0.0;
+    if (a >= 0)
+      for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)yj[(long)a * m + Lr[row * KC + q].idx];
+    if (c >= 0)
+      for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)yj[(long)c * m + Ur[row * KC + q].idx];
+    rw[j * cs + row] = (V)((double)rw[j * cs + row] - acc);
+  }
 }
 
 // FP16 copy of a batch of m x m FP32 blocks with one power-of-two scale per row:
@@ -579,13 +659,13 @@ struct BcrGraph {
 struct tf_bcr {
   int n, m, N, single;
   Order order;
-  void* xw;    // n (line-major solution when the order is not natural)
+  void* xw;    // 2n (line-major solution when the order is not natural; two columns)
   // captured apply: gin -> gout replayed as one graph launch (per vector type)
   int use_graph;
-  void* gin;
-  void* gout;
+  void* gin;   // 2n
+  void* gout;  // 2n
   hipStream_t cap;
-  BcrGraph g64, g32;
+  BcrGraph g64[2], g32[2];   // [columns - 1]
   size_t es;
   void* D;    // N m^2: diagonal blocks, inverted in place (Y)
   void* Lm;   // N m^2: dense lower couplings (levels >= 1)
@@ -600,9 +680,9 @@ struct tf_bcr {
   int *nLr, *nUr, *nUt, *nLt;
   int* flags;  // [err, nperturb]
   int* host_flags;
-  void* rw;    // n (solve workspace)
-  void* tb;    // n
-  void* yb;    // n (level-0 Y_e r_e)
+  void* rw;    // 2n (solve workspace, up to two columns at stride n)
+  void* tb;    // 2n
+  void* yb;    // 2n (level-0 Y_e r_e)
   void* pool;  // device pointer/index pool
   std::vector<LevelPlan> plan;
   std::vector<BcrLevelDev> lev;
@@ -766,10 +846,11 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
   return 0;
 }
 
-// forward and backward sweeps over the levels, rows read through the block source B
-template <typename B, typename V>
+// forward and backward sweeps over the levels, rows read through the block
+// source B, for NC columns at stride cs in rw, tb, xo (and the handle's yb)
+template <typename B, typename V, int NC>
 void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&), B Lb(const BcrLevelDev&),
-           B Ub(const BcrLevelDev&), B Yb(const BcrLevelDev&), V* rw, V* tb, V* xo, hipStream_t st) {
+           B Ub(const BcrLevelDev&), B Yb(const BcrLevelDev&), V* rw, V* tb, V* xo, long cs, hipStream_t st) {
   const int m = h->m;
   const dim3 blk(256);
   auto gx = [m](int G) { return (unsigned)((m + rows_per_block(G) - 1) / rows_per_block(G)); };
@@ -782,54 +863,63 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
       // the level-0 couplings are sparse: y_e = Y_e r_e, then r_k -= L_k y_e1 + U_k y_e2
       // (one Y block per eliminated line instead of two multipliers per kept line)
       const int G = group_for(L.ne);
-      hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)rw, yb);
-      hipLaunchKernelGGL((k_fwd_sparse<V>), dim3(gs, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr, h->nLr, h->Ur,
-                         h->nUr, (const V*)yb, rw);
+      hipLaunchKernelGGL((k_bwd_x<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)rw, yb,
+                         cs);
+      hipLaunchKernelGGL((k_fwd_sparse<V, NC>), dim3(gs, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr, h->nLr, h->Ur,
+                         h->nUr, (const V*)yb, rw, cs);
     } else {
       const int G = group_for(L.nk);
-      hipLaunchKernelGGL((k_fwd<B, V>), dim3(gx(G), L.nk), blk, 0, st, m, G, vec, L.k, L.k1, L.k2, P1(L), P2(L), rw);
+      hipLaunchKernelGGL((k_fwd<B, V, NC>), dim3(gx(G), L.nk), blk, 0, st, m, G, vec, L.k, L.k1, L.k2, P1(L), P2(L),
+                         rw, cs);
     }
   }
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
     if (l > 0 && h->fused) {
-      hipLaunchKernelGGL((k_bwd_fused<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Yb(L),
-                         Lb(L), Ub(L), (const V*)rw, xo);
+      hipLaunchKernelGGL((k_bwd_fused<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Yb(L),
+                         Lb(L), Ub(L), (const V*)rw, xo, cs);
       continue;
     }
     if (l == 0)
-      hipLaunchKernelGGL((k_bwd_t_sparse<V>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr, h->Ur,
-                         h->nUr, (const V*)rw, (const V*)xo, tb);
+      hipLaunchKernelGGL((k_bwd_t_sparse<V, NC>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr, h->Ur,
+                         h->nUr, (const V*)rw, (const V*)xo, tb, cs);
     else
-      hipLaunchKernelGGL((k_bwd_t<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Lb(L), Ub(L),
-                         (const V*)rw, (const V*)xo, tb);
-    hipLaunchKernelGGL((k_bwd_x<B, V>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)tb, xo);
+      hipLaunchKernelGGL((k_bwd_t<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Lb(L), Ub(L),
+                         (const V*)rw, (const V*)xo, tb, cs);
+    hipLaunchKernelGGL((k_bwd_x<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)tb, xo,
+                       cs);
   }
 }
 
-template <typename T, typename V>
-void levels(tf_bcr* h, V* rw, V* tb, V* xo, hipStream_t st) {
-  // 16-byte loads need m a multiple of the vector width and 16-byte aligned vectors
+// 16-byte row loads need m a multiple of the vector width and every column of
+// the vectors 16-byte aligned
+inline bool aligned16(const void* p, long cs, int nc, size_t es) {
+  return ((uintptr_t)p & 15) == 0 && (nc == 1 || (cs * (long)es) % 16 == 0);
+}
+
+template <typename T, typename V, int NC>
+void levels(tf_bcr* h, V* rw, V* tb, V* xo, long cs, hipStream_t st) {
   constexpr int W = 16 / (int)sizeof(T);
-  const int vec = (sizeof(T) == sizeof(V) && h->m % W == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
+  const int vec = (sizeof(T) == sizeof(V) && h->m % W == 0 && aligned16(xo, cs, NC, sizeof(V))) ? 1 : 0;
   typedef Blocks<T> B;
-  sweep<B, V>(
+  sweep<B, V, NC>(
       h, vec, [](const BcrLevelDev& L) { return B{(T* const*)L.P1}; },
       [](const BcrLevelDev& L) { return B{(T* const*)L.P2}; }, [](const BcrLevelDev& L) { return B{(T* const*)L.Le}; },
       [](const BcrLevelDev& L) { return B{(T* const*)L.Ue}; }, [](const BcrLevelDev& L) { return B{(T* const*)L.Ye}; },
-      rw, tb, xo, st);
+      rw, tb, xo, cs, st);
 }
 
-void levels_half(tf_bcr* h, float* rw, float* tb, float* xo, hipStream_t st) {
-  const int vec = (h->m % 8 == 0 && ((uintptr_t)xo & 15) == 0) ? 1 : 0;
+template <int NC>
+void levels_half(tf_bcr* h, float* rw, float* tb, float* xo, long cs, hipStream_t st) {
+  const int vec = (h->m % 8 == 0 && aligned16(xo, cs, NC, sizeof(float))) ? 1 : 0;
   typedef HalfBlocks B;
-  sweep<B, float>(
+  sweep<B, float, NC>(
       h, vec, [](const BcrLevelDev& L) { return B{(f16* const*)L.P1h, (float* const*)L.P1s}; },
       [](const BcrLevelDev& L) { return B{(f16* const*)L.P2h, (float* const*)L.P2s}; },
       [](const BcrLevelDev& L) { return B{(f16* const*)L.Lh, (float* const*)L.Ls}; },
       [](const BcrLevelDev& L) { return B{(f16* const*)L.Uh, (float* const*)L.Us}; },
-      [](const BcrLevelDev& L) { return B{(f16* const*)L.Yh, (float* const*)L.Ys}; }, rw, tb, xo, st);
+      [](const BcrLevelDev& L) { return B{(f16* const*)L.Yh, (float* const*)L.Ys}; }, rw, tb, xo, cs, st);
 }
 
 template <typename T, typename V>
@@ -837,10 +927,12 @@ double solve_work(const tf_bcr* h) {
   return (double)((sizeof(T) == 4 && sizeof(V) == 4 && h->half) ? h->solve_bytes_half : h->solve_bytes);
 }
 
-// prof: time the apply with the library profiler's events (never inside a capture)
-template <typename T, typename V>
-int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st, bool prof = true) {
-  const int m = h->m;
+// NC right-hand sides r[j] -> solutions x[j] (host arrays of device pointers).  The
+// work vectors hold the NC columns at stride n.  prof: time the apply with the
+// library profiler's events (never inside a capture).
+template <typename T, typename V, int NC>
+int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st, bool prof = true) {
+  const long n = h->n;
   V* rw = (V*)h->rw;
   V* tb = (V*)h->tb;
   if (prof) {
@@ -848,35 +940,55 @@ int solve_impl(tf_bcr* h, const V* r, V* x, hipStream_t st, bool prof = true) {
     TF_PROF_BEGIN(tf::PROF_SOLVE, st);
   }
   const bool nat = !h->order.ylines;
-  V* xo = nat ? x : (V*)h->xw;   // the kernels work in line-major order
-  if (nat)
-    hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, rw);
-  else
-    hipLaunchKernelGGL(k_to_lines<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, r, rw);
+  // one column in natural order: the kernels write the solution in place
+  V* xo = (nat && NC == 1) ? x[0] : (V*)h->xw;
+  for (int j = 0; j < NC; ++j) {
+    if (nat)
+      hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, r[j], rw + j * n);
+    else
+      hipLaunchKernelGGL(k_to_lines<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, r[j], rw + j * n);
+  }
   bool half = false;
   if constexpr (sizeof(T) == 4 && sizeof(V) == 4) half = h->half != 0;
-  if (half)
-    levels_half(h, (float*)rw, (float*)tb, (float*)xo, st);
-  else
-    levels<T, V>(h, rw, tb, xo, st);
-  if (!nat) hipLaunchKernelGGL(k_to_natural<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, (const V*)xo, x);
+  if constexpr (sizeof(T) == 4 && sizeof(V) == 4) {
+    if (half) levels_half<NC>(h, (float*)rw, (float*)tb, (float*)xo, n, st);
+  }
+  if (!half) levels<T, V, NC>(h, rw, tb, xo, n, st);
+  if (xo != x[0] || NC > 1) {
+    for (int j = 0; j < NC; ++j) {
+      if (nat)
+        hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, (const V*)(xo + j * n), x[j]);
+      else
+        hipLaunchKernelGGL(k_to_natural<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, (const V*)(xo + j * n),
+                           x[j]);
+    }
+  }
   if (prof) TF_PROF_END(tf::PROF_SOLVE, st);
   TF_LAUNCH_CHECK("bcr solve");
   return 0;
 }
 
-// The apply captured once per vector type on a private stream (gin -> gout) and
-// replayed with one graph launch; the factor refactors in place, so the
-// captured pointers stay valid.  The graph holds kernel nodes only.
-template <typename T, typename V>
-int solve_graph(tf_bcr* h, const V* r, V* x, hipStream_t st) {
-  BcrGraph& gs = sizeof(V) == 8 ? h->g64 : h->g32;
+// The apply captured once per (vector type, column count) on a private stream
+// (gin -> gout) and replayed with one graph launch; the factor refactors in
+// place, so the captured pointers stay valid.  The graph holds kernel nodes only.
+template <typename T, typename V, int NC>
+int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st) {
+  BcrGraph& gs = (sizeof(V) == 8 ? h->g64 : h->g32)[NC - 1];
+  const long n = h->n;
+  V* gin = (V*)h->gin;
+  V* gout = (V*)h->gout;
   if (!gs.exec) {
     // non-blocking: a capture must not order against the legacy stream that other
     // threads use for synchronous copies
     if (!h->cap) TF_HIP(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+    const V* ri[NC];
+    V* xi[NC];
+    for (int j = 0; j < NC; ++j) {
+      ri[j] = gin + j * n;
+      xi[j] = gout + j * n;
+    }
     TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-    const int e = solve_impl<T, V>(h, (const V*)h->gin, (V*)h->gout, h->cap, false);
+    const int e = solve_impl<T, V, NC>(h, ri, xi, h->cap, false);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(h->cap, &g);
     if (e) {
@@ -893,15 +1005,23 @@ int solve_graph(tf_bcr* h, const V* r, V* x, hipStream_t st) {
     gs.graph = g;
     gs.exec = ex;
   }
-  // the profiler's events bracket what an apply costs on this path: both staging copies and the replay
+  // the profiler's events bracket what an apply costs on this path: the staging copies and the replay
   TF_PROF_WORK(tf::PROF_SOLVE, (solve_work<T, V>(h)));
   TF_PROF_BEGIN(tf::PROF_SOLVE, st);
-  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, r, (V*)h->gin);
+  for (int j = 0; j < NC; ++j) hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, r[j], gin + j * n);
   TF_HIP(hipGraphLaunch(gs.exec, st));
-  hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, (long)h->n, (const V*)h->gout, x);
+  for (int j = 0; j < NC; ++j)
+    hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, (const V*)(gout + j * n), x[j]);
   TF_PROF_END(tf::PROF_SOLVE, st);
   TF_LAUNCH_CHECK("bcr graph apply");
   return 0;
+}
+
+template <typename V, int NC>
+int solve_any(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st) {
+  if (h->use_graph)
+    return h->single ? solve_graph<float, V, NC>(h, r, x, st) : solve_graph<double, V, NC>(h, r, x, st);
+  return h->single ? solve_impl<float, V, NC>(h, r, x, st) : solve_impl<double, V, NC>(h, r, x, st);
 }
 
 }  // namespace
@@ -987,12 +1107,12 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   for (int** c : {&h->nLr, &h->nUr, &h->nUt, &h->nLt}) TF_HIP(hipMalloc(c, sizeof(int) * (size_t)n));
   TF_HIP(hipMalloc(&h->flags, sizeof(int) * 2));
   TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
-  TF_HIP(hipMalloc(&h->rw, sizeof(double) * (size_t)n));
-  TF_HIP(hipMalloc(&h->tb, sizeof(double) * (size_t)n));
-  TF_HIP(hipMalloc(&h->yb, sizeof(double) * (size_t)n));
-  TF_HIP(hipMalloc(&h->xw, sizeof(double) * (size_t)n));
-  TF_HIP(hipMalloc(&h->gin, sizeof(double) * (size_t)n));
-  TF_HIP(hipMalloc(&h->gout, sizeof(double) * (size_t)n));
+  TF_HIP(hipMalloc(&h->rw, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->tb, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->xw, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->gin, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->gout, 2 * sizeof(double) * (size_t)n));   // two columns
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -1226,7 +1346,7 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
 
 int tf_bcr_destroy(tf_bcr* h) {
   if (!h) return 0;
-  for (BcrGraph* g : {&h->g64, &h->g32}) {
+  for (BcrGraph* g : {&h->g64[0], &h->g64[1], &h->g32[0], &h->g32[1]}) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
   }
@@ -1385,11 +1505,13 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
     h->cv_scl = (void**)(P + occ);
     h->ncv = (int)cs.size();
   }
-  if ((on != 0) != (h->half != 0) && h->g32.exec) {
-    // the captured FP32 apply reads the other storage: capture it again
-    (void)hipGraphExecDestroy(h->g32.exec);
-    (void)hipGraphDestroy(h->g32.graph);
-    h->g32 = BcrGraph{};
+  if ((on != 0) != (h->half != 0)) {
+    // the captured FP32 applies read the other storage: capture them again
+    for (BcrGraph& g : h->g32) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+      g = BcrGraph{};
+    }
   }
   h->half = on ? 1 : 0;
   if (h->half && h->factored && h->ncv > 0) {
@@ -1424,18 +1546,31 @@ int tf_bcr_set_fused(tf_bcr* h, int on) {
 
 int tf_bcr_solve_f64(tf_bcr* h, const double* r, double* x, void* stream) {
   if (!h) return tf::fail("null handle");
-  hipStream_t st = (hipStream_t)stream;
-  if (h->use_graph)
-    return h->single ? solve_graph<float, double>(h, r, x, st) : solve_graph<double, double>(h, r, x, st);
-  return h->single ? solve_impl<float, double>(h, r, x, st) : solve_impl<double, double>(h, r, x, st);
+  const double* rr[1] = {r};
+  double* xx[1] = {x};
+  return solve_any<double, 1>(h, rr, xx, (hipStream_t)stream);
 }
 
 int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream) {
   if (!h) return tf::fail("null handle");
-  hipStream_t st = (hipStream_t)stream;
-  if (h->use_graph)
-    return h->single ? solve_graph<float, float>(h, r, x, st) : solve_graph<double, float>(h, r, x, st);
-  return h->single ? solve_impl<float, float>(h, r, x, st) : solve_impl<double, float>(h, r, x, st);
+  const float* rr[1] = {r};
+  float* xx[1] = {x};
+  return solve_any<float, 1>(h, rr, xx, (hipStream_t)stream);
+}
+
+// Two right-hand sides in one pass over the factor blocks (x0 = M r0, x1 = M r1).
+int tf_bcr_solve2_f64(tf_bcr* h, const double* r0, const double* r1, double* x0, double* x1, void* stream) {
+  if (!h) return tf::fail("null handle");
+  const double* rr[2] = {r0, r1};
+  double* xx[2] = {x0, x1};
+  return solve_any<double, 2>(h, rr, xx, (hipStream_t)stream);
+}
+
+int tf_bcr_solve2_f32(tf_bcr* h, const float* r0, const float* r1, float* x0, float* x1, void* stream) {
+  if (!h) return tf::fail("null handle");
+  const float* rr[2] = {r0, r1};
+  float* xx[2] = {x0, x1};
+  return solve_any<float, 2>(h, rr, xx, (hipStream_t)stream);
 }
 
 }  // extern "C"

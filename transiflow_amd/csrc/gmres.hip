@@ -189,6 +189,196 @@ int poll_wait(hipStream_t st) {
   }
 }
 
+// ---- restarted FGMRES, one or two columns ---------------------------------------
+
+// The state of one system's FGMRES (its Krylov handle, right-hand side, iterate,
+// Hessenberg matrix, Givens rotations and counters).
+struct Column {
+  tf_krylov* k = nullptr;
+  const void* b = nullptr;
+  void* x = nullptr;
+  double bnrm = 0.0, rel = INFINITY, tol = 0.0;
+  int its = 0, kk = 0;
+  bool first = true, done = false, in_cycle = false, stepping = false;
+  std::vector<double> H, cs, sn, g, yk;
+  double& Hij(int i, int j) { return H[(size_t)i * k->m + j]; }
+};
+
+// b's norm; a zero right-hand side has the solution zero
+int col_start(Column& c, hipStream_t st) {
+  const int n = c.k->n, m = c.k->m;
+  c.H.assign((size_t)(m + 1) * m, 0.0);
+  c.cs.assign(m, 0.0);
+  c.sn.assign(m, 0.0);
+  c.g.assign(m + 1, 0.0);
+  c.yk.assign(m, 0.0);
+  if (int e = norm(c.k, n, c.b, &c.bnrm, st)) return e;
+  c.rel = 0.0;
+  if (c.bnrm == 0.0) {
+    c.done = true;
+    return c.k->single ? tf_fill_f32(n, 0.0, (float*)c.x, st) : tf_fill_f64(n, 0.0, (double*)c.x, st);
+  }
+  c.rel = INFINITY;
+  return 0;
+}
+
+// A restart cycle's first vector V[0] = r / |r|, or done (converged, out of
+// iterations, or a zero residual after the first cycle).
+int col_begin_cycle(Column& c, const int* rp, const int* ci, const void* val, int gauge, int maxit, int x_is_zero,
+                    hipStream_t st) {
+  const double tol = c.tol;
+  tf_krylov* k = c.k;
+  const int n = k->n;
+  Ops o{k, n, st};
+  double beta;
+  if (c.first && x_is_zero) {
+    TF_HIP(hipMemcpyAsync(o.V(0), c.b, k->es * (size_t)n, hipMemcpyDeviceToDevice, st));
+    beta = c.bnrm;
+  } else {
+    if (int e = spmv(k, n, rp, ci, val, gauge, c.x, k->w, st)) return e;
+    if (int e = k->single
+                    ? tf_axpby_f32(n, 1.0, (const float*)c.b, -1.0, (const float*)k->w, (float*)o.V(0), st)
+                    : tf_axpby_f64(n, 1.0, (const double*)c.b, -1.0, (const double*)k->w, (double*)o.V(0), st))
+      return e;
+    if (int e = norm(k, n, o.V(0), &beta, st)) return e;
+  }
+  c.rel = beta / c.bnrm;
+  if (c.rel <= tol || c.its >= maxit || (!c.first && beta == 0.0)) {
+    c.done = true;
+    return 0;
+  }
+  c.first = false;
+  if (int e = k->single ? tf_scale_f32(n, 1.0 / beta, (const float*)o.V(0), (float*)o.V(0), st)
+                        : tf_scale_f64(n, 1.0 / beta, (const double*)o.V(0), (double*)o.V(0), st))
+    return e;
+  std::fill(c.H.begin(), c.H.end(), 0.0);
+  std::fill(c.g.begin(), c.g.end(), 0.0);
+  c.g[0] = beta;
+  c.kk = 0;
+  c.in_cycle = true;
+  c.stepping = true;
+  return 0;
+}
+
+// Column j of the Hessenberg matrix (in k->h_host) through the previous Givens
+// rotations and a new one; stepping stops on convergence of the residual
+// estimate, the iteration limit, a breakdown or a non-finite estimate.
+void col_givens(Column& c, int j, int maxit) {
+  const double tol = c.tol;
+  for (int i = 0; i <= j + 1; ++i) c.Hij(i, j) = c.k->h_host[i];
+  for (int i = 0; i < j; ++i) {
+    const double t = c.cs[i] * c.Hij(i, j) + c.sn[i] * c.Hij(i + 1, j);
+    c.Hij(i + 1, j) = -c.sn[i] * c.Hij(i, j) + c.cs[i] * c.Hij(i + 1, j);
+    c.Hij(i, j) = t;
+  }
+  const double d = std::hypot(c.Hij(j, j), c.Hij(j + 1, j));
+  if (d == 0.0) {
+    c.kk = j;
+    c.stepping = false;
+    return;
+  }
+  c.cs[j] = c.Hij(j, j) / d;
+  c.sn[j] = c.Hij(j + 1, j) / d;
+  c.Hij(j, j) = d;
+  c.Hij(j + 1, j) = 0.0;
+  c.g[j + 1] = -c.sn[j] * c.g[j];
+  c.g[j] = c.cs[j] * c.g[j];
+  ++c.its;
+  c.kk = j + 1;
+  if (std::fabs(c.g[j + 1]) / c.bnrm <= tol || c.its >= maxit || !std::isfinite(c.g[j + 1])) c.stepping = false;
+}
+
+// x += Z y with H y = g (upper triangular); a single-precision solve that ran out
+// of iterations ends on its residual estimate.
+int col_end_cycle(Column& c, int maxit, hipStream_t st) {
+  tf_krylov* k = c.k;
+  const int n = k->n, kk = c.kk;
+  c.in_cycle = false;
+  c.stepping = false;
+  if (kk == 0) {
+    c.done = true;
+    return 0;
+  }
+  for (int i = kk - 1; i >= 0; --i) {
+    double sum = c.g[i];
+    for (int q = i + 1; q < kk; ++q) sum -= c.Hij(i, q) * c.yk[q];
+    c.yk[i] = sum / c.Hij(i, i);
+  }
+  TF_HIP(hipMemcpyAsync(k->y, c.yk.data(), sizeof(double) * kk, hipMemcpyHostToDevice, st));
+  if (int e = k->single ? tf_multiaxpy_f32(n, kk, (const float*)k->Z, n, k->y, (float*)c.x, 1.0, st)
+                        : tf_multiaxpy_f64(n, kk, (const double*)k->Z, n, k->y, (double*)c.x, 1.0, st))
+    return e;
+  if (c.its >= maxit && k->single) {
+    c.rel = std::fabs(c.g[kk]) / c.bnrm;
+    c.done = true;
+  }
+  return 0;
+}
+
+// The preconditioner on the stepping columns' V[j] -> Z[j]: one two-column apply of
+// the line factor when two columns step together, else one apply each.
+int precond_columns(Column* c, int nc, int j, int kind, void* prec, int ncell, int dof, hipStream_t st) {
+  int act[2], na = 0;
+  for (int q = 0; q < nc; ++q)
+    if (c[q].stepping) act[na++] = q;
+  if (na == 2 && kind == 1) {
+    tf_krylov* k0 = c[act[0]].k;
+    tf_krylov* k1 = c[act[1]].k;
+    Ops o0{k0, k0->n, st}, o1{k1, k1->n, st};
+    return k0->single ? tf_bcr_solve2_f32((tf_bcr*)prec, (const float*)o0.V(j), (const float*)o1.V(j),
+                                          (float*)o0.Z(j), (float*)o1.Z(j), st)
+                      : tf_bcr_solve2_f64((tf_bcr*)prec, (const double*)o0.V(j), (const double*)o1.V(j),
+                                          (double*)o0.Z(j), (double*)o1.Z(j), st);
+  }
+  for (int a = 0; a < na; ++a) {
+    tf_krylov* k = c[act[a]].k;
+    Ops o{k, k->n, st};
+    if (int e = precond(k, kind, prec, ncell, dof, o.V(j), o.Z(j), st)) return e;
+  }
+  return 0;
+}
+
+int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* val, int gauge, int prec_kind,
+                   void* prec, int ncell, int dof, int maxit, int x_is_zero, hipStream_t st) {
+  for (int q = 0; q < nc; ++q)
+    if (int e = col_start(c[q], st)) return e;
+  // the Arnoldi step graphs read the matrix through each handle's descriptor
+  for (int q = 0; q < nc; ++q)
+    hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, st, c[q].k->desc, rp, ci, val, gauge);
+  TF_LAUNCH_CHECK("fgmres descriptor");
+  const int m = c[0].k->m;
+  for (;;) {
+    bool any = false;
+    for (int q = 0; q < nc; ++q) {
+      if (c[q].done) continue;
+      if (int e = col_begin_cycle(c[q], rp, ci, val, gauge, maxit, x_is_zero, st)) return e;
+      any = any || c[q].in_cycle;
+    }
+    if (!any) break;
+    for (int j = 0; j < m; ++j) {
+      bool stepping = false;
+      for (int q = 0; q < nc; ++q) stepping = stepping || c[q].stepping;
+      if (!stepping) break;
+      if (int e = precond_columns(c, nc, j, prec_kind, prec, ncell, dof, st)) return e;
+      for (int q = 0; q < nc; ++q) {
+        if (!c[q].stepping) continue;
+        tf_krylov* k = c[q].k;
+        if (int e = (k->use_graph && !tf::prof_on()) ? arnoldi_step_graph(k, j, st) : arnoldi_step(k, j, st))
+          return e;
+      }
+      if (int e = poll_wait(st)) return e;
+      for (int q = 0; q < nc; ++q)
+        if (c[q].stepping) col_givens(c[q], j, maxit);
+    }
+    for (int q = 0; q < nc; ++q)
+      if (c[q].in_cycle)
+        if (int e = col_end_cycle(c[q], maxit, st)) return e;
+  }
+  // the host copies of y must stay valid until the multiaxpys have read them
+  TF_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,100 +433,54 @@ int tf_krylov_destroy(tf_krylov* k) {
 
 // Solve (gauged A) x = b by FGMRES with A in CSR (values in the handle's precision),
 // preconditioner kind 0 none / 1 tf_bcr / 2 tf_linelu / 3 cell-block inverse (handle =
-// the FP64 inverse blocks, ncell, dof).  x holds the initial guess (x_is_zero: it is
-// zero and the first residual is b).  its_out / rel_out: iterations and the final
-// relative residual -- the true one, except for a single-precision solve that ends on
-// the iteration limit (the residual estimate then; callers refine in FP64).
+// the FP64 inverse blocks, ncell, dof) / 4 tf_mg / 5 tf_lineilu.  x holds the initial
+// guess (x_is_zero: it is zero and the first residual is b).  its_out / rel_out:
+// iterations and the final relative residual -- the true one, except for a
+// single-precision solve that ends on the iteration limit (the residual estimate
+// then; callers refine in FP64).
 int tf_fgmres(tf_krylov* k, const int* rp, const int* ci, const void* val, int gauge, int prec_kind,
               void* prec, int ncell, int dof, const void* b, void* x, double tol, int maxit, int x_is_zero,
               int* its_out, double* rel_out, void* stream) {
   if (!k || !its_out || !rel_out) return tf::fail("null argument");
-  hipStream_t st = (hipStream_t)stream;
-  const int n = k->n, m = k->m;
-  Ops o{k, n, st};
-  double bnrm = 0.0;
-  if (int e = norm(k, n, b, &bnrm, st)) return e;
-  *its_out = 0;
-  *rel_out = 0.0;
-  if (bnrm == 0.0)
-    return k->single ? tf_fill_f32(n, 0.0, (float*)x, st) : tf_fill_f64(n, 0.0, (double*)x, st);
-  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yk(m);
-  // the Arnoldi step graphs read the matrix through the descriptor
-  hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, st, k->desc, rp, ci, val, gauge);
-  TF_LAUNCH_CHECK("fgmres descriptor");
-  auto Hij = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
-  int its = 0;
-  double rel = INFINITY;
-  bool first = true;
-  for (;;) {
-    double beta;
-    if (first && x_is_zero) {
-      TF_HIP(hipMemcpyAsync(o.V(0), b, k->es * (size_t)n, hipMemcpyDeviceToDevice, st));
-      beta = bnrm;
-    } else {
-      if (int e = spmv(k, n, rp, ci, val, gauge, x, k->w, st)) return e;
-      if (int e = k->single ? tf_axpby_f32(n, 1.0, (const float*)b, -1.0, (const float*)k->w, (float*)o.V(0), st)
-                            : tf_axpby_f64(n, 1.0, (const double*)b, -1.0, (const double*)k->w, (double*)o.V(0), st))
-        return e;
-      if (int e = norm(k, n, o.V(0), &beta, st)) return e;
-    }
-    rel = beta / bnrm;
-    if (rel <= tol || its >= maxit || (!first && beta == 0.0)) break;
-    first = false;
-    if (int e = k->single ? tf_scale_f32(n, 1.0 / beta, (const float*)o.V(0), (float*)o.V(0), st)
-                          : tf_scale_f64(n, 1.0 / beta, (const double*)o.V(0), (double*)o.V(0), st))
-      return e;
-    std::fill(H.begin(), H.end(), 0.0);
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int kk = 0;
-    bool estimate_exit = false;
-    for (int j = 0; j < m; ++j) {
-      if (int e = precond(k, prec_kind, prec, ncell, dof, o.V(j), o.Z(j), st)) return e;
-      if (int e = (k->use_graph && !tf::prof_on()) ? arnoldi_step_graph(k, j, st) : arnoldi_step(k, j, st)) return e;
-      if (int e = poll_wait(st)) return e;
-      for (int i = 0; i <= j + 1; ++i) Hij(i, j) = k->h_host[i];
-      for (int i = 0; i < j; ++i) {
-        const double t = cs[i] * Hij(i, j) + sn[i] * Hij(i + 1, j);
-        Hij(i + 1, j) = -sn[i] * Hij(i, j) + cs[i] * Hij(i + 1, j);
-        Hij(i, j) = t;
-      }
-      const double d = std::hypot(Hij(j, j), Hij(j + 1, j));
-      if (d == 0.0) {
-        kk = j;
-        break;
-      }
-      cs[j] = Hij(j, j) / d;
-      sn[j] = Hij(j + 1, j) / d;
-      Hij(j, j) = d;
-      Hij(j + 1, j) = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      ++its;
-      kk = j + 1;
-      if (std::fabs(g[j + 1]) / bnrm <= tol || its >= maxit || !std::isfinite(g[j + 1])) break;
-    }
-    if (kk == 0) break;
-    for (int i = kk - 1; i >= 0; --i) {
-      double s = g[i];
-      for (int c = i + 1; c < kk; ++c) s -= Hij(i, c) * yk[c];
-      yk[i] = s / Hij(i, i);
-    }
-    TF_HIP(hipMemcpyAsync(k->y, yk.data(), sizeof(double) * kk, hipMemcpyHostToDevice, st));
-    if (int e = k->single ? tf_multiaxpy_f32(n, kk, (const float*)k->Z, n, k->y, (float*)x, 1.0, st)
-                          : tf_multiaxpy_f64(n, kk, (const double*)k->Z, n, k->y, (double*)x, 1.0, st))
-      return e;
-    if (its >= maxit && k->single) {
-      rel = std::fabs(g[kk]) / bnrm;
-      estimate_exit = true;
-    }
-    if (estimate_exit) break;
+  Column c[1];
+  c[0].k = k;
+  c[0].b = b;
+  c[0].x = x;
+  c[0].tol = tol;
+  const int e = fgmres_columns(c, 1, rp, ci, val, gauge, prec_kind, prec, ncell, dof, maxit, x_is_zero,
+                               (hipStream_t)stream);
+  *its_out = c[0].its;
+  *rel_out = c[0].rel;
+  return e;
+}
+
+// Two systems with the same matrix and preconditioner (A x0 = b0, A x1 = b1), each
+// with its own Krylov handle (same size, restart and precision), solved in lockstep:
+// while both are in a restart cycle each preconditioner application serves both
+// (tf_bcr_solve2_*: one pass over the factor blocks).  Each column's iterates are
+// those of tf_fgmres on its own system with its own tolerance.
+int tf_fgmres2(tf_krylov* k0, tf_krylov* k1, const int* rp, const int* ci, const void* val, int gauge,
+               int prec_kind, void* prec, int ncell, int dof, const void* b0, const void* b1, void* x0, void* x1,
+               double tol0, double tol1, int maxit, int x_is_zero, int* its_out, double* rel_out, void* stream) {
+  if (!k0 || !k1 || !its_out || !rel_out) return tf::fail("null argument");
+  if (k0 == k1 || k0->n != k1->n || k0->m != k1->m || k0->single != k1->single)
+    return tf::fail("tf_fgmres2: two distinct Krylov handles of one size, restart and precision needed");
+  Column c[2];
+  c[0].k = k0;
+  c[0].b = b0;
+  c[0].x = x0;
+  c[1].k = k1;
+  c[1].b = b1;
+  c[1].x = x1;
+  c[0].tol = tol0;
+  c[1].tol = tol1;
+  const int e = fgmres_columns(c, 2, rp, ci, val, gauge, prec_kind, prec, ncell, dof, maxit, x_is_zero,
+                               (hipStream_t)stream);
+  for (int j = 0; j < 2; ++j) {
+    its_out[j] = c[j].its;
+    rel_out[j] = c[j].rel;
   }
-  // the host copy of y must stay valid until the multiaxpy has read it
-  TF_HIP(hipStreamSynchronize(st));
-  *its_out = its;
-  *rel_out = rel;
-  return 0;
+  return e;
 }
 
 }  // extern "C"

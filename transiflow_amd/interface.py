@@ -174,6 +174,11 @@ class Interface:
     def solve(self, jac, rhs, rhs2=None, V=None, W=None, C=None):
         return self.solver.solve(jac, rhs, rhs2, V, W, C)
 
+    def solve_pair(self, jac, rhs0, rhs1):
+        """(solve(jac, rhs0), solve(jac, rhs1)), the two solves of a Newton-corrector
+        iteration (Continuation.py:158-159), solved together (Solver.solve_pair)."""
+        return self.solver.solve_pair(jac, rhs0, rhs1)
+
     # -- persistence (BaseInterface.py:118-215) ---------------------------
     def save_json(self, name, obj):
         with open(name, 'w') as f:

@@ -38,6 +38,10 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Step Graphs'``      replay each native FGMRES Arnoldi step (SpMV, two CGS
                        passes, normalisation) as one captured hipGraph (True)
                        or launch its kernels one by one;
+``'Pair Solves'``      (True) ``solve_pair`` -- the two solves of a Newton-
+                       corrector iteration -- runs both in lockstep with one
+                       pass over the line factor per application for both;
+                       False: two solves;
 ``'Apply Storage'``    (mixed, line factor) ``'fp16'`` (default) or ``'fp32'``:
                        with ``'fp16'`` the inner FP32 apply reads FP16 copies of
                        the factor blocks (one FP32 scale per row), half the bytes
@@ -88,7 +92,8 @@ iterate instead of raising (``'Fail On Tolerance'`` False); the reference's
 here (the preconditioner is chosen as above).
 
 Every solve appends (iterations, relative residual, preconditioner name,
-factorised?) to ``Solver.log`` for measurement.
+factorisations it made -- a bool, or a count for the first solve of a pair) to
+``Solver.log`` for measurement.
 """
 
 import ctypes
@@ -131,6 +136,7 @@ def _opts(parameters):
         'Learned Network': o.get('Learned Network'),
         'Apply Graph': bool(o.get('Apply Graph', True)),
         'Step Graphs': bool(o.get('Step Graphs', True)),
+        'Pair Solves': bool(o.get('Pair Solves', True)),
         'Apply Storage': o.get('Apply Storage', 'fp16'),
         'Background Factor': bool(o.get('Background Factor', False)),
         'Factor GEMM': o.get('Factor GEMM', 'fp32'),
@@ -244,6 +250,17 @@ class LineFactor:
     def apply32(self, r, out):
         _lib.call(self._p + '_solve_f32', self._h, ptr(r), ptr(out), stream())
         return out
+
+    def apply_pair(self, r0, r1, out0, out1, single=False):
+        """Two right-hand sides: one pass over the blocks of the block cyclic
+        reduction (tf_bcr_solve2_*), else two applies."""
+        if self._p == 'tf_bcr':
+            _lib.call('tf_bcr_solve2_f32' if single else 'tf_bcr_solve2_f64', self._h, ptr(r0), ptr(r1), ptr(out0),
+                      ptr(out1), stream())
+        else:
+            (self.apply32 if single else self.apply)(r0, out0)
+            (self.apply32 if single else self.apply)(r1, out1)
+        return out0, out1
 
     def set_half(self, on):
         """FP32 apply from FP16 copies of the blocks (block cyclic reduction, FP32 factor only)."""
@@ -611,6 +628,17 @@ class NativeKrylov:
                   ctypes.byref(rel), stream())
         return its.value, rel.value
 
+    def solve2(self, other, jac, val, gauge, prec, b0, b1, x0, x1, tol0, tol1, maxit, x_is_zero=False):
+        """Two systems with one matrix and preconditioner in lockstep (tf_fgmres2);
+        ``other`` is a second NativeKrylov of the same size, restart and precision."""
+        kind, handle, ncell, dof = prec.native()
+        its = (ctypes.c_int * 2)()
+        rel = (ctypes.c_double * 2)()
+        _lib.call('tf_fgmres2', self._h, other._h, ptr(jac.row_ptr), ptr(jac.col), ptr(val), int(gauge), kind, handle,
+                  ncell, dof, ptr(b0), ptr(b1), ptr(x0), ptr(x1), float(tol0), float(tol1), int(maxit),
+                  1 if x_is_zero else 0, its, rel, stream())
+        return (its[0], rel[0]), (its[1], rel[1])
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and h.value:
@@ -754,8 +782,8 @@ class Solver:
         self._factors = {}    # (kind, single) -> LineFactor, refactored in place
         self._bg = None       # BackgroundFactor (mixed, 'Background Factor')
 
-    def _krylov_for(self, restart, dtype='float64', native=False):
-        key = (dtype, native)
+    def _krylov_for(self, restart, dtype='float64', native=False, slot=0):
+        key = (dtype, native, slot)
         k = self._krylov.get(key)
         if k is None or k.m != restart:
             k = self._krylov[key] = (NativeKrylov if native else Krylov)(self.itf.n, restart, dtype)
@@ -1082,6 +1110,183 @@ class Solver:
             rel = _norm(r, work) / bnrm
         self.refinements.append(outer)
         return total, rel, prec, fresh
+
+    def solve_pair(self, jac, rhs0, rhs1):
+        """(y0, y1) with (gauged J) y = rhs for two right-hand sides and one Jacobian:
+        the two solves of a Newton-corrector iteration (Continuation.py:158-159).
+        Solved together where the preconditioner is the native line factor: the two
+        FGMRES runs advance in lockstep and every preconditioner application
+        serves both (tf_fgmres2 / tf_bcr_solve2_*: one pass over the factor
+        blocks for two vectors); in mixed precision the FP64 refinement of the two
+        runs in lockstep too.  Each result meets 'Tolerance' as a single solve
+        does, and each is logged as a solve.  Otherwise: two solves."""
+        n = self.itf.n
+        T = torch()
+        for r in (rhs0, rhs1):
+            if not isinstance(r, DeviceVector) or r.size != n:
+                raise ValueError('solve_pair needs two DeviceVectors of %d entries' % n)
+        opts = _opts(self.itf.parameters)
+        mixed = opts['Precision'] == 'mixed'
+        # the learned correction runs in the single solves until it has switched itself off
+        learned = opts['Learned Network'] is not None and not (self._learned is not None and self._learned.off)
+        if (not opts['Pair Solves'] or learned or self.observer is not None
+                or self._background(opts) is not None or (not mixed and opts['Reuse Factor'])):
+            return self.solve(jac, rhs0), self.solve(jac, rhs1)
+        prec, fresh = self.preconditioner(jac, opts)
+        if not hasattr(prec, 'native') or prec.native()[0] != 1:
+            return self.solve(jac, rhs0), self.solve(jac, rhs1)
+        g = self.gauge()
+        bs = [rhs0.data.clone(), rhs1.data.clone()]
+        xs = [T.empty_like(bs[0]), T.empty_like(bs[0])]
+        for b, x in zip(bs, xs):
+            if g >= 0:
+                _lib.call('tf_set_entry_f64', ptr(b), b.numel(), g, 0.0, stream())
+            _lib.call('tf_fill_f64', x.numel(), 0.0, ptr(x), stream())
+        res = (self._solve_mixed_pair if mixed else self._solve_fp64_pair)(jac, bs, xs, opts, prec, fresh)
+        for its, rel, p, fr in res:
+            self.log.append((its, rel, getattr(p, 'name', '?'), fr))
+        for its, rel, _, _ in res:
+            if not rel <= opts['Tolerance'] and opts['Fail On Tolerance']:
+                raise SolverError('GMRES did not reach the tolerance: relative residual %.3e after %d iterations'
+                                  % (rel, its))
+        return DeviceVector(xs[0]), DeviceVector(xs[1])
+
+    def _solve_fp64_pair(self, jac, bs, xs, opts, prec, fresh):
+        g = self.gauge()
+        k0 = self._krylov_for(opts['Restart'], native=True)
+        k1 = self._krylov_for(opts['Restart'], native=True, slot=1)
+        (i0, r0), (i1, r1) = k0.solve2(k1, jac, jac.val, g, prec, bs[0], bs[1], xs[0], xs[1], opts['Tolerance'],
+                                       opts['Tolerance'], opts['Maximum Iterations'], x_is_zero=True)
+        return [(i0, r0, prec, fresh), (i1, r1, prec, False)]
+
+    def _pair_buffers(self, n):
+        T = torch()
+        if getattr(self, '_buf2', None) is None or self._buf2[0][0].numel() != n:
+            self._buf2 = tuple((T.empty(n, dtype=T.float64, device='cuda'), T.empty(n, dtype=T.float32, device='cuda'),
+                                T.empty(n, dtype=T.float32, device='cuda'), T.empty(n, dtype=T.float64, device='cuda'))
+                               for _ in range(2))
+        return self._buf2
+
+    def _solve_mixed_pair(self, jac, bs, xs, opts, prec, fresh):
+        """The FP64 iterative refinement of _solve_mixed for two right-hand sides in
+        lockstep: each refinement round computes both FP64 residuals, solves both
+        corrections with one tf_fgmres2 (FP32, lagged FP32 factor), and applies
+        the factor-reuse rules once for the pair (a refactorisation serves both)."""
+        n = self.itf.n
+        g = self.gauge()
+        st = stream()
+        bufs = self._pair_buffers(n)
+        k0 = self._krylov_for(opts['Restart'], 'float32', native=True)
+        k1 = self._krylov_for(opts['Restart'], 'float32', native=True, slot=1)
+        work = self._krylov_for(opts['Restart']).work
+        val32 = jac.values32()
+        factored = 1 if fresh else 0   # factorisations made for this pair (logged with the first column)
+        tol, itol0 = opts['Tolerance'], opts['Inner Tolerance']
+        inner_max = min(opts['Restart'], opts['Inner Iterations'])
+        cols = []
+        for c in range(2):
+            bnrm = _norm(bs[c], work)
+            cols.append({'bnrm': bnrm, 'total': 0, 'outer': 0, 'rel': 1.0 if bnrm > 0 else 0.0, 'prev': math.inf,
+                         'done': bnrm == 0.0, 'prec': prec})
+        while True:
+            live = [c for c in range(2) if not cols[c]['done']]
+            for c in live:
+                C = cols[c]
+                if C['outer'] >= opts['Maximum Refinements'] or C['total'] >= opts['Maximum Iterations']:
+                    C['done'] = True
+            live = [c for c in range(2) if not cols[c]['done']]
+            if not live:
+                break
+            refactor = False
+            for c in live:
+                C = cols[c]
+                r, r32, d32, d = bufs[c]
+                if C['outer'] > 0:
+                    jac.matvec(xs[c], out=r, gauge=g)
+                    _lib.call('tf_axpby_f64', n, 1.0, ptr(bs[c]), -1.0, ptr(r), ptr(r), st)
+                    C['rel'] = _norm(r, work) / C['bnrm']
+                    if C['rel'] <= tol:
+                        C['done'] = True
+                        continue
+                    if C['rel'] > 0.5 * C['prev'] and not jac.prec_exact:
+                        refactor = True     # the lagged factor no longer contracts
+                    elif C['rel'] > 0.5 * C['prev'] and jac.prec_exact:
+                        # stagnation with this Jacobian's own FP32 factor: finish in FP64
+                        its64, C['rel'], p64, _ = self._fp64_fallback(jac, bs[c], xs[c], opts)
+                        C['total'] += its64
+                        C['prec'] = p64
+                        self.fallbacks += 1
+                        C['done'] = True
+                        continue
+                    C['prev'] = C['rel']
+            live = [c for c in range(2) if not cols[c]['done']]
+            if not live:
+                break
+            if refactor:
+                self.stale_refactors += 1
+                prec, fr = self._refactor_now(jac, opts)
+                factored += 1 if fr else 0
+                for c in live:
+                    cols[c]['prev'] = math.inf
+                    cols[c]['prec'] = prec
+            itols = {}
+            for c in live:
+                C = cols[c]
+                r, r32, d32, d = bufs[c]
+                _lib.call('tf_f64_to_f32', n, ptr(r if C['outer'] > 0 else bs[c]), ptr(r32), st)
+                _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
+                itol = itol0
+                if C['outer'] > 0:
+                    need = 0.5 * tol / C['rel']
+                    if INNER_FLOOR <= need < itol:
+                        itol = need
+                itols[c] = itol
+            if len(live) == 2:
+                (i0, e0), (i1, e1) = k0.solve2(k1, jac, val32, g, prec, bufs[0][1], bufs[1][1], bufs[0][2],
+                                               bufs[1][2], itols[0], itols[1], inner_max, x_is_zero=True)
+                inner = {0: (i0, e0), 1: (i1, e1)}
+            else:
+                c = live[0]
+                inner = {c: k0.solve(jac, val32, g, prec, bufs[c][1], bufs[c][2], itols[c], inner_max,
+                                     x_is_zero=True)}
+            fresh_its = []
+            for c in live:
+                C = cols[c]
+                r, r32, d32, d = bufs[c]
+                its, est = inner[c]
+                self.trace.append((its, est, C['rel']))
+                C['total'] += its
+                _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)
+                _lib.call('tf_axpby_f64', n, 1.0, ptr(xs[c]), 1.0, ptr(d), ptr(xs[c]), st)
+                C['outer'] += 1
+                if jac.prec_exact and C['outer'] == 1 and itols[c] == itol0:
+                    fresh_its.append(its)
+            if fresh_its:
+                prec.fresh_inner_its = max(fresh_its)
+            limit_hit = False
+            for c in live:
+                base = max(opts['Reuse Limit'], 1.5 * getattr(prec, 'fresh_inner_its', 0))
+                limit = base * math.log(itols[c]) / math.log(itol0)
+                if not jac.prec_exact and inner[c][0] > limit:
+                    limit_hit = True
+            if limit_hit:
+                self.limit_refactors += 1
+                prec, fr = self._refactor_now(jac, opts)
+                factored += 1 if fr else 0
+                for c in live:
+                    cols[c]['prev'] = math.inf
+                    cols[c]['prec'] = prec
+        out = []
+        for c in range(2):
+            C = cols[c]
+            r, _, _, _ = bufs[c]
+            if C['bnrm'] > 0 and not C['rel'] <= tol:
+                jac.matvec(xs[c], out=r, gauge=g)
+                _lib.call('tf_axpby_f64', n, 1.0, ptr(bs[c]), -1.0, ptr(r), ptr(r), st)
+                C['rel'] = _norm(r, work) / C['bnrm']
+            self.refinements.append(C['outer'])
+            out.append((C['total'], C['rel'], C['prec'], factored if c == 0 else 0))
+        return out
 
     def solve_complex(self, A, rhs):
         """y with (gauged A) y = rhs for a complex matrix A = Ar + i Ai (the shifted
