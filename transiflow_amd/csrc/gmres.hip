@@ -20,6 +20,7 @@
 #include "tf_common.h"
 #include "transiflow_hip.h"
 
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -48,8 +49,22 @@ struct tf_krylov {
   hipStream_t cap;           // private stream the step graphs are captured on
   hipGraphExec_t step[MAX_STEPS];
   hipGraph_t step_graph[MAX_STEPS];
+  // two-system steps with the partner handle step2_partner (captured on this handle)
+  hipGraphExec_t step2[MAX_STEPS];
+  hipGraph_t step2_graph[MAX_STEPS];
+  unsigned long long uid;            // unique per handle (a freed handle's address can come back)
+  unsigned long long step2_partner;  // uid of the partner the step2 graphs were captured with
   int use_graph;
 };
+
+namespace tf {
+int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, double* w1, double* work0,
+                  double* work1, double* h0, double* h1, double* y0, double* y1, double* v0, double* v1,
+                  hipStream_t st);
+int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
+                  double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
+                  hipStream_t st);
+}  // namespace tf
 
 namespace {
 
@@ -64,6 +79,16 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_spmv_desc(int n, const MatrixDesc* __restrict__ d, const T* __restrict__ x,
                                                    T* __restrict__ y) {
   tf::spmv(n, d->rp, d->ci, (const T*)d->val, x, y, d->gauge);
+}
+
+// the SpMV of two systems (blockIdx.y picks descriptor and vectors)
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_desc2(int n, const MatrixDesc* __restrict__ d0,
+                                                    const T* __restrict__ x0, T* __restrict__ y0,
+                                                    const MatrixDesc* __restrict__ d1, const T* __restrict__ x1,
+                                                    T* __restrict__ y1) {
+  const MatrixDesc* d = blockIdx.y ? d1 : d0;
+  tf::spmv(n, d->rp, d->ci, (const T*)d->val, blockIdx.y ? x1 : x0, blockIdx.y ? y1 : y0, d->gauge);
 }
 
 struct Ops {
@@ -177,6 +202,68 @@ int arnoldi_step_graph(tf_krylov* k, int j, hipStream_t st) {
     k->step[j] = ex;
   }
   TF_HIP(hipGraphLaunch(k->step[j], st));
+  return 0;
+}
+
+// Arnoldi step j of two systems with handles a and b (same n, restart and precision)
+// in the same launches: both SpMVs, both CGS2 orthogonalisations and
+// normalisations (tf::cgs2_pair_*), each Hessenberg column to its handle's
+// h_host.  The arithmetic is that of arnoldi_step on each.
+int arnoldi_step2(tf_krylov* a, tf_krylov* b, int j, hipStream_t st) {
+  const int n = a->n;
+  Ops oa{a, n, st}, ob{b, n, st};
+  if (a->single)
+    hipLaunchKernelGGL(k_spmv_desc2<float>, dim3(tf::spmv_blocks<float>(n), 2), dim3(256), 0, st, n, a->desc,
+                       (const float*)oa.Z(j), (float*)a->w, b->desc, (const float*)ob.Z(j), (float*)b->w);
+  else
+    hipLaunchKernelGGL(k_spmv_desc2<double>, dim3(tf::spmv_blocks<double>(n), 2), dim3(256), 0, st, n, a->desc,
+                       (const double*)oa.Z(j), (double*)a->w, b->desc, (const double*)ob.Z(j), (double*)b->w);
+  TF_LAUNCH_CHECK("fgmres spmv pair");
+  if (int e = a->single ? tf::cgs2_pair_f32(n, j, (const float*)a->V, (const float*)b->V, (float*)a->w, (float*)b->w,
+                                            a->work, b->work, a->h, b->h, a->y, b->y, (float*)oa.V(j + 1),
+                                            (float*)ob.V(j + 1), st)
+                        : tf::cgs2_pair_f64(n, j, (const double*)a->V, (const double*)b->V, (double*)a->w,
+                                            (double*)b->w, a->work, b->work, a->h, b->h, a->y, b->y,
+                                            (double*)oa.V(j + 1), (double*)ob.V(j + 1), st))
+    return e;
+  TF_HIP(hipMemcpyAsync(a->h_host, a->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  TF_HIP(hipMemcpyAsync(b->h_host, b->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  return 0;
+}
+
+// The two-system step j replayed from a graph captured on handle a (recaptured
+// when a's partner changes).
+int arnoldi_step2_graph(tf_krylov* a, tf_krylov* b, int j, hipStream_t st) {
+  if (a->step2_partner != b->uid) {
+    for (int q = 0; q < MAX_STEPS; ++q) {
+      if (a->step2[q]) (void)hipGraphExecDestroy(a->step2[q]);
+      if (a->step2_graph[q]) (void)hipGraphDestroy(a->step2_graph[q]);
+      a->step2[q] = nullptr;
+      a->step2_graph[q] = nullptr;
+    }
+    a->step2_partner = b->uid;
+  }
+  if (!a->step2[j]) {
+    if (!a->cap) TF_HIP(hipStreamCreateWithFlags(&a->cap, hipStreamNonBlocking));
+    TF_HIP(hipStreamBeginCapture(a->cap, hipStreamCaptureModeThreadLocal));
+    const int e = arnoldi_step2(a, b, j, a->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(a->cap, &g);
+    if (e) {
+      if (g) (void)hipGraphDestroy(g);
+      return e;
+    }
+    TF_HIP(ce);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return tf::fail_hip(ie, "hipGraphInstantiate (two-system Arnoldi step)");
+    }
+    a->step2_graph[j] = g;
+    a->step2[j] = ex;
+  }
+  TF_HIP(hipGraphLaunch(a->step2[j], st));
   return 0;
 }
 
@@ -360,11 +447,18 @@ int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* 
       for (int q = 0; q < nc; ++q) stepping = stepping || c[q].stepping;
       if (!stepping) break;
       if (int e = precond_columns(c, nc, j, prec_kind, prec, ncell, dof, st)) return e;
-      for (int q = 0; q < nc; ++q) {
-        if (!c[q].stepping) continue;
-        tf_krylov* k = c[q].k;
-        if (int e = (k->use_graph && !tf::prof_on()) ? arnoldi_step_graph(k, j, st) : arnoldi_step(k, j, st))
+      if (nc == 2 && c[0].stepping && c[1].stepping) {
+        tf_krylov* a = c[0].k;
+        tf_krylov* b = c[1].k;
+        if (int e = (a->use_graph && !tf::prof_on()) ? arnoldi_step2_graph(a, b, j, st) : arnoldi_step2(a, b, j, st))
           return e;
+      } else {
+        for (int q = 0; q < nc; ++q) {
+          if (!c[q].stepping) continue;
+          tf_krylov* k = c[q].k;
+          if (int e = (k->use_graph && !tf::prof_on()) ? arnoldi_step_graph(k, j, st) : arnoldi_step(k, j, st))
+            return e;
+        }
       }
       if (int e = poll_wait(st)) return e;
       for (int q = 0; q < nc; ++q)
@@ -404,6 +498,8 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
     return 0;
   };
   k->use_graph = 1;
+  static std::atomic<unsigned long long> next_uid{1};
+  k->uid = next_uid++;
   if (int e = alloc()) {
     tf_krylov_destroy(k);  // frees the partial set (pointers start null)
     return e;
@@ -423,6 +519,8 @@ int tf_krylov_destroy(tf_krylov* k) {
   for (int j = 0; j < MAX_STEPS; ++j) {
     if (k->step[j]) (void)hipGraphExecDestroy(k->step[j]);
     if (k->step_graph[j]) (void)hipGraphDestroy(k->step_graph[j]);
+    if (k->step2[j]) (void)hipGraphExecDestroy(k->step2[j]);
+    if (k->step2_graph[j]) (void)hipGraphDestroy(k->step2_graph[j]);
   }
   if (k->cap) (void)hipStreamDestroy(k->cap);
   for (void* p : {k->V, k->Z, k->w, (void*)k->h, (void*)k->y, (void*)k->work, (void*)k->desc}) (void)hipFree(p);

@@ -55,10 +55,20 @@ int md_blocks(long n) {
 // wavefront shuffles and then across the block's waves in a fixed order.
 // VEC: 16-byte loads over entries [b0, b1) counted in vectors (rows of V and w
 // 16-byte aligned, n a multiple of the vector width); else one entry per lane.
+// A grid with gridDim.y == 2 does the same for a second system: blockIdx.y 1
+// takes V1, w1 and partial1 (the other kernels below follow the same rule).
 template <typename T, bool VEC>
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, const T* __restrict__ V, long ldv,
                                                                  const T* __restrict__ w,
-                                                                 double* __restrict__ partial) {
+                                                                 double* __restrict__ partial,
+                                                                 const T* __restrict__ V1 = nullptr,
+                                                                 const T* __restrict__ w1 = nullptr,
+                                                                 double* __restrict__ partial1 = nullptr) {
+  if (blockIdx.y) {
+    V = V1;
+    w = w1;
+    partial = partial1;
+  }
   using Acc = typename std::conditional<sizeof(T) == 8, double, float>::type;
   using VT = typename Vec16<T>::type;
   constexpr int VN = VEC ? Vec16<T>::N : 1;
@@ -110,7 +120,13 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_partial(int n, int k, c
 
 // h[i] (+)= sum_b partial[i][b]; one workgroup per i, wavefront reductions.
 __global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, int nb, const double* __restrict__ partial,
-                                                               double* __restrict__ h, int accumulate) {
+                                                               double* __restrict__ h, int accumulate,
+                                                               const double* __restrict__ partial1 = nullptr,
+                                                               double* __restrict__ h1 = nullptr) {
+  if (blockIdx.y) {
+    partial = partial1;
+    h = h1;
+  }
   __shared__ double sh[MD_WAVES];
   const int i = blockIdx.x;
   double acc = 0.0;
@@ -130,7 +146,15 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, int nb, co
 // loads issued together.
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __restrict__ V, long ldv,
-                                                   const double* __restrict__ coef, T* __restrict__ w, double sign) {
+                                                   const double* __restrict__ coef, T* __restrict__ w, double sign,
+                                                   const T* __restrict__ V1 = nullptr,
+                                                   const double* __restrict__ coef1 = nullptr,
+                                                   T* __restrict__ w1 = nullptr) {
+  if (blockIdx.y) {
+    V = V1;
+    coef = coef1;
+    w = w1;
+  }
   using VT = typename Vec16<T>::type;
   constexpr int VN = VEC ? Vec16<T>::N : 1;
   __shared__ double c[MAX_BASIS];
@@ -172,7 +196,14 @@ __global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __rest
 // out = w / nrm where nrm is a device scalar (the next Arnoldi vector).
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void k_scale_by_inv(int n, const T* __restrict__ w, const double* __restrict__ nrm,
-                                                      T* __restrict__ out) {
+                                                      T* __restrict__ out, const T* __restrict__ w1 = nullptr,
+                                                      const double* __restrict__ nrm1 = nullptr,
+                                                      T* __restrict__ out1 = nullptr) {
+  if (blockIdx.y) {
+    w = w1;
+    nrm = nrm1;
+    out = out1;
+  }
   using VT = typename Vec16<T>::type;
   constexpr int VN = VEC ? Vec16<T>::N : 1;
   const double inv = 1.0 / *nrm;
@@ -191,7 +222,13 @@ __global__ __launch_bounds__(256) void k_scale_by_inv(int n, const T* __restrict
 
 // sqrt(sum partial[0..nb)) into out
 __global__ __launch_bounds__(MD_THREADS) void k_norm_final(int nb, const double* __restrict__ partial,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out,
+                                                           const double* __restrict__ partial1 = nullptr,
+                                                           double* __restrict__ out1 = nullptr) {
+  if (blockIdx.y) {
+    partial = partial1;
+    out = out1;
+  }
   __shared__ double sh[MD_WAVES];
   double acc = 0.0;
   for (int b = threadIdx.x; b < nb; b += MD_THREADS) acc += partial[b];
@@ -370,7 +407,78 @@ int normalize(int n, const T* w, double* work, double* nrm_dev, T* out, hipStrea
   return 0;
 }
 
+// h[i] += y[i], i <= j, for two systems (blockIdx.x 0 and 1)
+__global__ void k_add_pair(int k, double* h0, const double* y0, double* h1, const double* y1) {
+  double* h = blockIdx.x ? h1 : h0;
+  const double* y = blockIdx.x ? y1 : y0;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) h[i] = h[i] + y[i];
+}
+
+template <typename T>
+int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work0, double* work1, double* h0,
+              double* h1, double* y0, double* y1, T* v0, T* v1, hipStream_t st) {
+  const int k = j + 1;
+  if (k < 1 || k + 1 > MAX_BASIS) return tf::fail("cgs2 pair: bad basis size");
+  const int nb = md_blocks(n);
+  const bool vec = vec_ok<T>(n, n, V0, w0) && vec_ok<T>(n, n, V1, w1) && vec_ok<T>(n, n, v0, v1);
+  const dim3 md(nb, 2), blk(MD_THREADS), fin(k, 2);
+  const dim3 ew(grid_for(vec ? n / Vec16<T>::N : n), 2);
+  for (int pass = 0; pass < 2; ++pass) {
+    double* c0 = pass ? y0 : h0;
+    double* c1 = pass ? y1 : h1;
+    if (vec)
+      hipLaunchKernelGGL((k_multidot_partial<T, true>), md, blk, 0, st, n, k, V0, (long)n, (const T*)w0, work0, V1,
+                         (const T*)w1, work1);
+    else
+      hipLaunchKernelGGL((k_multidot_partial<T, false>), md, blk, 0, st, n, k, V0, (long)n, (const T*)w0, work0, V1,
+                         (const T*)w1, work1);
+    hipLaunchKernelGGL(k_multidot_final, fin, blk, 0, st, k, nb, (const double*)work0, c0, 0, (const double*)work1,
+                       c1);
+    if (vec)
+      hipLaunchKernelGGL((k_multiaxpy<T, true>), ew, dim3(256), 0, st, n, k, V0, (long)n, (const double*)c0, w0, -1.0,
+                         V1, (const double*)c1, w1);
+    else
+      hipLaunchKernelGGL((k_multiaxpy<T, false>), ew, dim3(256), 0, st, n, k, V0, (long)n, (const double*)c0, w0,
+                         -1.0, V1, (const double*)c1, w1);
+  }
+  hipLaunchKernelGGL(k_add_pair, dim3(2), dim3(64), 0, st, k, h0, (const double*)y0, h1, (const double*)y1);
+  // |w| into h[j+1], V[j+1] = w / |w|
+  if (vec)
+    hipLaunchKernelGGL((k_multidot_partial<T, true>), md, blk, 0, st, n, 1, (const T*)w0, (long)n, (const T*)w0,
+                       work0, (const T*)w1, (const T*)w1, work1);
+  else
+    hipLaunchKernelGGL((k_multidot_partial<T, false>), md, blk, 0, st, n, 1, (const T*)w0, (long)n, (const T*)w0,
+                       work0, (const T*)w1, (const T*)w1, work1);
+  hipLaunchKernelGGL(k_norm_final, dim3(1, 2), blk, 0, st, nb, (const double*)work0, h0 + k, (const double*)work1,
+                     h1 + k);
+  if (vec)
+    hipLaunchKernelGGL((k_scale_by_inv<T, true>), ew, dim3(256), 0, st, n, (const T*)w0, (const double*)(h0 + k), v0,
+                       (const T*)w1, (const double*)(h1 + k), v1);
+  else
+    hipLaunchKernelGGL((k_scale_by_inv<T, false>), ew, dim3(256), 0, st, n, (const T*)w0, (const double*)(h0 + k),
+                       v0, (const T*)w1, (const double*)(h1 + k), v1);
+  TF_LAUNCH_CHECK("cgs2 pair");
+  return 0;
+}
+
 }  // namespace
+
+namespace tf {
+// One CGS2 Arnoldi orthogonalisation and normalisation for two systems in the
+// same launches (each kernel's blockIdx.y picks the system): for each,
+// h = V^T w, w -= V h, y = V^T w, w -= V y, h += y, h[j+1] = |w|, V[j+1] = w / |w|,
+// with the arithmetic of the single-system building blocks.
+int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, double* w1, double* work0,
+                  double* work1, double* h0, double* h1, double* y0, double* y1, double* v0, double* v1,
+                  hipStream_t st) {
+  return cgs2_pair<double>(n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+}
+int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
+                  double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
+                  hipStream_t st) {
+  return cgs2_pair<float>(n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+}
+}  // namespace tf
 
 extern "C" {
 

@@ -1095,7 +1095,10 @@ class Solver:
                 prec.fresh_inner_its = its
             base = max(opts['Reuse Limit'], 1.5 * getattr(prec, 'fresh_inner_its', 0))
             limit = base * math.log(itol) / math.log(opts['Inner Tolerance'])
-            if not jac.prec_exact and its > limit:
+            # (a lagged factor whose inner solve ran out of iterations short of the
+            # inner tolerance is stale too, also when the limit is above the cap)
+            capped = its >= inner_max and est > itol and itol == opts['Inner Tolerance']
+            if not jac.prec_exact and (its > limit or capped):
                 # lagged factor too stale: refactor this Jacobian and keep refining
                 # (in the background: with the lagged factor until the new one is ready)
                 self.limit_refactors += 1
@@ -1267,7 +1270,8 @@ class Solver:
             for c in live:
                 base = max(opts['Reuse Limit'], 1.5 * getattr(prec, 'fresh_inner_its', 0))
                 limit = base * math.log(itols[c]) / math.log(itol0)
-                if not jac.prec_exact and inner[c][0] > limit:
+                capped = inner[c][0] >= inner_max and inner[c][1] > itols[c] and itols[c] == itol0
+                if not jac.prec_exact and (inner[c][0] > limit or capped):
                     limit_hit = True
             if limit_hit:
                 self.limit_refactors += 1
