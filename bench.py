@@ -408,7 +408,8 @@ def kernel_sample(n, reps=10):
     """Short launch sequence for hardware-counter runs: one FP32 line
     factorisation of the n x n cavity Jacobian at Re = 1000, ``reps`` FP32
     applies reading the FP16 copies of the factor blocks (the default inner
-    apply of the mixed-precision solver), and a plain copy of known size
+    apply of the mixed-precision solver), ``reps`` two-column applies (the
+    paired corrector solves), and a plain copy of known size
     (out = x - y over 64 Mi floats: 512 MiB read, 256 MiB written) to calibrate
     FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -422,13 +423,17 @@ def kernel_sample(n, reps=10):
     z = torch.empty_like(r)
     for _ in range(reps):
         P.apply32(r, z)
+    # the two-column apply of the paired corrector solves
+    r1, z1 = torch.full_like(r, 0.5), torch.empty_like(r)
+    for _ in range(reps):
+        P.apply_pair(r, r1, z, z1, single=True)
     big = 64 * 2 ** 20
     a = torch.ones(big, dtype=torch.float32, device='cuda')
     b = torch.ones(big, dtype=torch.float32, device='cuda')
     c = torch.empty(big, dtype=torch.float32, device='cuda')
     _lib.call('tf_axpby_f32', big, 1.0, ptr(a), -1.0, ptr(b), ptr(c), stream())
     torch.cuda.synchronize()
-    log('kernel sample done: %d applies, %d-float copy' % (reps, big))
+    log('kernel sample done: %d applies, %d two-column applies, %d-float copy' % (reps, reps, big))
 
 
 def conv_sample(reps=5, n=64):

@@ -32,7 +32,7 @@ Options (``parameters['HIP Solver']``, all optional):
                        refinement does the rest); a step whose remaining
                        reduction to ``'Tolerance'`` is at least 1e-5 asks for
                        half of it instead, and so finishes the solve;
-``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (10);
+``'Inner Iterations'`` (mixed) FGMRES iterations per inner solve (12);
 ``'Apply Graph'``      replay the line-factor apply as one captured hipGraph
                        (True);
 ``'Step Graphs'``      replay each native FGMRES Arnoldi step (SpMV, two CGS
@@ -56,7 +56,7 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Restart'``          Krylov basis size (30);
 ``'Maximum Iterations'`` (300);
 ``'Reuse Limit'``      (mixed) GMRES iterations above which a lagged
-                       preconditioner is refactored (8, for 'Inner Tolerance';
+                       preconditioner is refactored (10, for 'Inner Tolerance';
                        scaled by the reduction a step asks for; at least 1.5 times
                        the iterations the preconditioner needed on the Jacobian it
                        was made from); a refinement step that
@@ -68,7 +68,7 @@ Options (``parameters['HIP Solver']``, all optional):
 ``'Reuse Factor'``     (fp64) False (default) or True: reuse the factor of an
                        earlier Jacobian as the mixed precision does, refactoring
                        when a solve takes more than 'Reuse Limit' iterations
-                       scaled from 'Inner Tolerance' to 'Tolerance' (24 by
+                       scaled from 'Inner Tolerance' to 'Tolerance' (30 by
                        default) or does not converge;
 ``'Fused Apply'``      (line factor) True or False (default: True for a factor
                        that is reused -- mixed, or 'Reuse Factor' -- else
@@ -129,9 +129,9 @@ def _opts(parameters):
         'Restart': int(o.get('Restart', ip.get('Restart', 100) if it else 30)),
         'Maximum Iterations': int(o.get('Maximum Iterations', ip.get('Maximum Iterations', 1000) if it else 300)),
         'Fail On Tolerance': bool(o.get('Fail On Tolerance', not it)),
-        'Reuse Limit': int(o.get('Reuse Limit', 8)),
+        'Reuse Limit': int(o.get('Reuse Limit', 10)),
         'Inner Tolerance': float(o.get('Inner Tolerance', 5e-4)),
-        'Inner Iterations': int(o.get('Inner Iterations', 10)),
+        'Inner Iterations': int(o.get('Inner Iterations', 12)),
         'Maximum Refinements': int(o.get('Maximum Refinements', 20)),
         'Learned Network': o.get('Learned Network'),
         'Apply Graph': bool(o.get('Apply Graph', True)),
@@ -1137,7 +1137,10 @@ class Solver:
             return self.solve(jac, rhs0), self.solve(jac, rhs1)
         prec, fresh = self.preconditioner(jac, opts)
         if not hasattr(prec, 'native') or prec.native()[0] != 1:
-            return self.solve(jac, rhs0), self.solve(jac, rhs1)
+            y0 = self.solve(jac, rhs0)
+            if fresh:   # the factor made above served this solve
+                self.log[-1] = self.log[-1][:3] + (True,)
+            return y0, self.solve(jac, rhs1)
         g = self.gauge()
         bs = [rhs0.data.clone(), rhs1.data.clone()]
         xs = [T.empty_like(bs[0]), T.empty_like(bs[0])]
