@@ -109,11 +109,13 @@ def test_solve_mixed_bf16_factor_gemm(hip, shape, re, factor_gemm):
     assert itf.solver.fallbacks == 0
 
 
-@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1), (24, 17, 1)])
+@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1), (24, 17, 1), (100, 100, 1), (120, 104, 1)])
 def test_half_apply(hip, shape):
     """FP32 apply from FP16 copies of the factor blocks (one power-of-two FP32 scale
     per row): still an approximate inverse of the gauged Jacobian (residual of
-    A z = r at FP16 level), close to the apply from the FP32 blocks, bit-identical
+    A z = r at FP16 level), close to the apply from the FP32 blocks (100 x 100 and
+    120 x 104: the widest levels take two rows per lane group, with a ragged last
+    block; m = 300 reads element-wise, m = 312 / 360 with 16-byte loads), bit-identical
     between eager and graph replays and across switching the copies off and on, and
     remade by a refactorisation (the residual bar holds for a second Jacobian
     factored into the same handle)."""
@@ -158,7 +160,9 @@ def test_half_apply(hip, shape):
         print('%s state %d: residual FP16 copies %.2e, FP32 blocks %.2e, |z16 - z32|/|z32| %.2e'
               % (shape, k, res_h, res_32, diff), flush=True)
         assert numpy.isfinite(zh[0]).all()
-        assert res_h <= 1e-2
+        # the FP16 residual grows with the grid's conditioning: 4e-3 .. 8e-3 up to
+        # 48 x 48, 1.03e-2 at 100 x 100
+        assert res_h <= (1e-2 if n <= 48 * 48 * 3 else 2.5e-2)
         assert diff <= 1e-2
         if prev is not None:
             assert not numpy.array_equal(zh[0], prev)
@@ -348,13 +352,13 @@ def test_graph_apply_matches_eager(hip, shape):
         P.set_graph(False)
 
 
-@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1), (8, 8, 6)])
+@pytest.mark.parametrize('shape', [(48, 48, 1), (64, 40, 1), (8, 8, 6), (100, 100, 1)])
 @pytest.mark.parametrize('storage', ['fp64', 'fp32', 'fp16'])
 def test_pair_apply_matches_single(hip, shape, storage):
     """Two right-hand sides through one pass over the factor blocks give, bit for
     bit, the two single applies: x- and y-line orders (64 x 40 factors its
-    shorter y-lines) and 3D planes, FP64 / FP32 / FP16-copy blocks, eager and
-    replayed."""
+    shorter y-lines) and 3D planes, FP64 / FP32 / FP16-copy blocks (at 100 x 100
+    two rows per lane group at the widest levels), eager and replayed."""
     import torch
     itf = hip.Interface({'Reynolds Number': 400}, *shape)
     n = itf.n

@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -267,83 +268,105 @@ struct Vec16<double> {
 // the upper levels, where each group's chain of dependent loads sets the time.
 __host__ __device__ __forceinline__ int rows_per_block(int G) { return 256 / G; }
 inline int group_for(int batch) { return batch >= 32 ? 16 : (batch >= 8 ? 32 : 64); }
-
-// NC columns: sum over c of a[c] x_j[c] (x_j = x + j cs) by the G-lane group g
-// (every lane of the group gets the results).  The row is read once for all
-// columns; each column's sum has the order of the single-column case, so an NC
-// column apply equals NC single-column applies bit for bit.
-template <int NC, typename T, typename V>
-__device__ __forceinline__ void group_dot(const T* __restrict__ a, const V* __restrict__ x, long cs, int m, int g,
-                                          int G, bool vec, double (&out)[NC]) {
-  double acc[NC];
+// NC columns, R rows: sum over c of a_i[c] x_j[c] (x_j = x + j cs) by the G-lane
+// group g (every lane of the group gets the results).  Each row is read once for
+// all columns and each x chunk once for all rows; every (row, column) sum has the
+// order of the one-row, one-column case, so the result does not depend on NC or R
+// (an NC-column apply equals NC single-column applies bit for bit).
+template <int NC, int R, typename T, typename V>
+__device__ __forceinline__ void group_dot(const T* const (&a)[R], const V* __restrict__ x, long cs, int m, int g,
+                                          int G, bool vec, double (&out)[R][NC]) {
+  double acc[R][NC];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[i][j] = 0.0;
   if constexpr (sizeof(T) == sizeof(V)) {
     if (vec) {
       typedef typename Vec16<T>::type VT;
       constexpr int W = Vec16<T>::W;
-      const VT* av = (const VT*)a;
 #pragma unroll 4
       for (int c = g; c < m / W; c += G) {
-        const VT p = av[c];
+        VT p[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) p[i] = ((const VT*)a[i])[c];
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
           const VT q = ((const VT*)(x + j * cs))[c];
 #pragma unroll
-          for (int w = 0; w < W; ++w) acc[j] += (double)p[w] * (double)q[w];
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int w = 0; w < W; ++w) acc[i][j] += (double)p[i][w] * (double)q[w];
         }
       }
     } else {
       for (int c = g; c < m; c += G)
 #pragma unroll
-        for (int j = 0; j < NC; ++j) acc[j] += (double)a[c] * (double)x[j * cs + c];
+        for (int j = 0; j < NC; ++j)
+#pragma unroll
+          for (int i = 0; i < R; ++i) acc[i][j] += (double)a[i][c] * (double)x[j * cs + c];
     }
   } else {
     for (int c = g; c < m; c += G)
 #pragma unroll
-      for (int j = 0; j < NC; ++j) acc[j] += (double)a[c] * (double)x[j * cs + c];
+      for (int j = 0; j < NC; ++j)
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i][j] += (double)a[i][c] * (double)x[j * cs + c];
   }
 #pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, G);
-    out[j] = acc[j];
-  }
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      for (int off = G / 2; off > 0; off >>= 1) acc[i][j] += __shfl_xor(acc[i][j], off, G);
+      out[i][j] = acc[i][j];
+    }
 }
 
-// FP16 row against FP32 vectors: 8 halves (16 bytes) and two float4 per lane per step
-template <int NC>
-__device__ __forceinline__ void group_dot(const f16* __restrict__ a, const float* __restrict__ x, long cs, int m,
-                                          int g, int G, bool vec, double (&out)[NC]) {
-  double acc[NC];
+// FP16 rows against FP32 vectors: 8 halves (16 bytes) per row and two float4 per
+// column per lane per step
+template <int NC, int R>
+__device__ __forceinline__ void group_dot(const f16* const (&a)[R], const float* __restrict__ x, long cs, int m,
+                                          int g, int G, bool vec, double (&out)[R][NC]) {
+  double acc[R][NC];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[i][j] = 0.0;
   if (vec) {
-    const f16x8* av = (const f16x8*)a;
 #pragma unroll 2
     for (int c = g; c < m / 8; c += G) {
-      const f16x8 p = av[c];
+      f16x8 p[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) p[i] = ((const f16x8*)a[i])[c];
 #pragma unroll
       for (int j = 0; j < NC; ++j) {
         const f32x4* xv = (const f32x4*)(x + j * cs);
         const f32x4 q0 = xv[2 * c], q1 = xv[2 * c + 1];
-        float sv = 0.0f;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) sv = fmaf((float)p[w], q0[w], sv);
+        for (int i = 0; i < R; ++i) {
+          float sv = 0.0f;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) sv = fmaf((float)p[4 + w], q1[w], sv);
-        acc[j] += (double)sv;
+          for (int w = 0; w < 4; ++w) sv = fmaf((float)p[i][w], q0[w], sv);
+#pragma unroll
+          for (int w = 0; w < 4; ++w) sv = fmaf((float)p[i][4 + w], q1[w], sv);
+          acc[i][j] += (double)sv;
+        }
       }
     }
   } else {
     for (int c = g; c < m; c += G)
 #pragma unroll
-      for (int j = 0; j < NC; ++j) acc[j] += (double)(float)a[c] * (double)x[j * cs + c];
+      for (int j = 0; j < NC; ++j)
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i][j] += (double)(float)a[i][c] * (double)x[j * cs + c];
   }
 #pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, G);
-    out[j] = acc[j];
-  }
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      for (int off = G / 2; off > 0; off >>= 1) acc[i][j] += __shfl_xor(acc[i][j], off, G);
+      out[i][j] = acc[i][j];
+    }
 }
 
 // Rows of a batch of m x m blocks in the factor's storage type.
@@ -362,78 +385,138 @@ struct HalfBlocks {
   __device__ __forceinline__ double scale(int b, int r) const { return (double)s[b][r]; }
 };
 
-template <int NC, typename B, typename V>
-__device__ __forceinline__ void row_dot(const B& blk, int b, int r, const V* __restrict__ x, long cs, int m, int g,
-                                        int G, bool vec, double (&out)[NC]) {
-  group_dot<NC>(blk.row(b, r, m), x, cs, m, g, G, vec, out);
-  const double sc = blk.scale(b, r);
+// Rows per lane group R: with two, the rows share each load of the vector chunk.
+// Two for FP16 rows once a launch has 12288 rows (batch x m) or more, so that the
+// halved group count still fills the device; one for rows in the factor's own
+// type.  Measured on the line-factor apply (FP32 from FP16 blocks, one and two
+// columns): 256 x 256 cavity 294 / 388 -> 295 / 355 us, 32^3 cavity 649 / 914 ->
+// 627 / 735 us; two rows of FP64 blocks were slower at 32^3 (2296 -> 2438 us).
+template <typename B>
+int rows_for(int batch, int m) {
+  return (std::is_same<B, HalfBlocks>::value && (long)batch * m >= 12288) ? 2 : 1;
+}
+
+// f(std::integral_constant<int, R>) for the run-time row count R (1 or 2)
+template <typename F>
+void with_rows(int R, F&& f) {
+  if (R == 2)
+    f(std::integral_constant<int, 2>());
+  else
+    f(std::integral_constant<int, 1>());
+}
+
+template <int NC, int R, typename B, typename V>
+__device__ __forceinline__ void row_dot(const B& blk, int b, const int (&r)[R], const V* __restrict__ x, long cs,
+                                        int m, int g, int G, bool vec, double (&out)[R][NC]) {
+  decltype(blk.row(b, 0, m)) a[R];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) out[j] *= sc;
+  for (int i = 0; i < R; ++i) a[i] = blk.row(b, r[i], m);
+  group_dot<NC, R>(a, x, cs, m, g, G, vec, out);
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const double sc = blk.scale(b, r[i]);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) out[i][j] *= sc;
+  }
+}
+
+// The R rows of this lane group: a block covers R * rows_per_block(G) rows; rows
+// past m are clamped to row 0 (read, never written).
+template <int R>
+__device__ __forceinline__ void group_rows(int m, int G, int (&rr)[R], bool (&live)[R]) {
+  const int rpb = rows_per_block(G);
+  const int r0 = blockIdx.x * rpb * R + (int)threadIdx.x / G;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int r = r0 + i * rpb;
+    live[i] = r < m;
+    rr[i] = live[i] ? r : 0;
+  }
 }
 
 // Every vector argument below holds NC columns at stride cs.
 
 // rw[k] -= P1 rw[e1] + P2 rw[e2] (terms with a missing neighbour dropped) for a batch of kept lines
-template <typename B, typename V, int NC>
+template <typename B, typename V, int NC, int R>
 __global__ __launch_bounds__(256) void k_fwd(int m, int G, int vec, const int* __restrict__ k_line,
                                              const int* __restrict__ e1, const int* __restrict__ e2, B P1, B P2,
                                              V* __restrict__ rw, long cs) {
   const int b = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
-  const bool live = r < m;
-  const int rr = live ? r : 0;
-  double acc[NC], d[NC];
+  int rr[R];
+  bool live[R];
+  group_rows<R>(m, G, rr, live);
+  double acc[R][NC], d[R][NC];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[i][j] = 0.0;
   if (e1[b] >= 0) {
-    row_dot<NC>(P1, b, rr, rw + (long)e1[b] * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(P1, b, rr, rw + (long)e1[b] * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[i][j] += d[i][j];
   }
   if (e2[b] >= 0) {
-    row_dot<NC>(P2, b, rr, rw + (long)e2[b] * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(P2, b, rr, rw + (long)e2[b] * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += d[j];
-  }
-  if (live && g == 0) {
+    for (int i = 0; i < R; ++i)
 #pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      V* out = rw + j * cs + (long)k_line[b] * m + r;
-      *out = (V)((double)*out - acc[j]);
-    }
+      for (int j = 0; j < NC; ++j) acc[i][j] += d[i][j];
   }
+  if (g == 0)
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (live[i])
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          V* out = rw + j * cs + (long)k_line[b] * m + rr[i];
+          *out = (V)((double)*out - acc[i][j]);
+        }
 }
 
 // t[e] = rw[e] - L_e x[a] - U_e x[b]  (dense L/U, levels >= 1)
-template <typename B, typename V, int NC>
+template <typename B, typename V, int NC, int R>
 __global__ __launch_bounds__(256) void k_bwd_t(int m, int G, int vec, const int* __restrict__ e_line,
                                                const int* __restrict__ a_line, const int* __restrict__ b_line,
                                                B Lb, B Ub, const V* __restrict__ rw, const V* __restrict__ x,
                                                V* __restrict__ tbuf, long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
-  const bool live = r < m;
-  const int rr = live ? r : 0;
-  const long row = (long)e_line[bi] * m + rr;
+  int rr[R];
+  bool live[R];
+  group_rows<R>(m, G, rr, live);
+  const long base = (long)e_line[bi] * m;
   const int a = a_line[bi], bb = b_line[bi];
-  double acc[NC], d[NC];
+  double acc[R][NC], d[R][NC];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[i][j] = 0.0;
   if (a >= 0) {
-    row_dot<NC>(Lb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(Lb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[i][j] += d[i][j];
   }
   if (bb >= 0) {
-    row_dot<NC>(Ub, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(Ub, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[i][j] += d[i][j];
   }
-  if (live && g == 0)
+  if (g == 0)
 #pragma unroll
-    for (int j = 0; j < NC; ++j) tbuf[j * cs + row] = (V)((double)rw[j * cs + row] - acc[j]);
+    for (int i = 0; i < R; ++i)
+      if (live[i])
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          const long row = base + rr[i];
+          tbuf[j * cs + row] = (V)((double)rw[j * cs + row] - acc[i][j]);
+        }
 }
 
 // Level 0: t[e] = rw[e] - L_e x[a] - U_e x[b] with the sparse couplings.  Thread per row.
@@ -463,52 +546,62 @@ __global__ __launch_bounds__(256) void k_bwd_t_sparse(int m, const int* __restri
 }
 
 // x[e] = Y_e t[e]
-template <typename B, typename V, int NC>
+template <typename B, typename V, int NC, int R>
 __global__ __launch_bounds__(256) void k_bwd_x(int m, int G, int vec, const int* __restrict__ e_line, B Yb,
                                                const V* __restrict__ tbuf, V* __restrict__ x, long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
-  const bool live = r < m;
-  const int rr = live ? r : 0;
+  int rr[R];
+  bool live[R];
+  group_rows<R>(m, G, rr, live);
   const long base = (long)e_line[bi] * m;
-  double d[NC];
-  row_dot<NC>(Yb, bi, rr, tbuf + base, cs, m, g, G, vec, d);
-  if (live && g == 0)
+  double d[R][NC];
+  row_dot<NC, R>(Yb, bi, rr, tbuf + base, cs, m, g, G, vec, d);
+  if (g == 0)
 #pragma unroll
-    for (int j = 0; j < NC; ++j) x[j * cs + base + r] = (V)d[j];
+    for (int i = 0; i < R; ++i)
+      if (live[i])
+#pragma unroll
+        for (int j = 0; j < NC; ++j) x[j * cs + base + rr[i]] = (V)d[i][j];
 }
 
 // Levels >= 1 of a factor made with the products Y L and Y U in place of L and U
 // (tf_bcr_set_fused): x[e] = Y_e rw[e] - (Y_e L_e) x[a] - (Y_e U_e) x[b] in one
 // pass, the same three blocks read and no intermediate vector.
-template <typename B, typename V, int NC>
+template <typename B, typename V, int NC, int R>
 __global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const int* __restrict__ e_line,
                                                    const int* __restrict__ a_line, const int* __restrict__ b_line,
                                                    B Yb, B YLb, B YUb, const V* __restrict__ rw, V* __restrict__ x,
                                                    long cs) {
   const int bi = blockIdx.y;
   const int g = threadIdx.x % G;
-  const int r = blockIdx.x * rows_per_block(G) + threadIdx.x / G;
-  const bool live = r < m;
-  const int rr = live ? r : 0;
+  int rr[R];
+  bool live[R];
+  group_rows<R>(m, G, rr, live);
   const long base = (long)e_line[bi] * m;
   const int a = a_line[bi], bb = b_line[bi];
-  double acc[NC], d[NC];
-  row_dot<NC>(Yb, bi, rr, rw + base, cs, m, g, G, vec, acc);
+  double acc[R][NC], d[R][NC];
+  row_dot<NC, R>(Yb, bi, rr, rw + base, cs, m, g, G, vec, acc);
   if (a >= 0) {
-    row_dot<NC>(YLb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(YLb, bi, rr, x + (long)a * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[i][j] -= d[i][j];
   }
   if (bb >= 0) {
-    row_dot<NC>(YUb, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
+    row_dot<NC, R>(YUb, bi, rr, x + (long)bb * m, cs, m, g, G, vec, d);
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[i][j] -= d[i][j];
   }
-  if (live && g == 0)
+  if (g == 0)
 #pragma unroll
-    for (int j = 0; j < NC; ++j) x[j * cs + base + r] = (V)acc[j];
+    for (int i = 0; i < R; ++i)
+      if (live[i])
+#pragma unroll
+        for (int j = 0; j < NC; ++j) x[j * cs + base + rr[i]] = (V)acc[i][j];
 }
 
 // copy a batch of m x m blocks: dst[b] = src[b]   (grid.y = batch)
@@ -853,7 +946,7 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
            B Ub(const BcrLevelDev&), B Yb(const BcrLevelDev&), V* rw, V* tb, V* xo, long cs, hipStream_t st) {
   const int m = h->m;
   const dim3 blk(256);
-  auto gx = [m](int G) { return (unsigned)((m + rows_per_block(G) - 1) / rows_per_block(G)); };
+  auto gx = [m](int G, int R) { return (unsigned)((m + R * rows_per_block(G) - 1) / (R * rows_per_block(G))); };
   const unsigned gs = (unsigned)((m + 255) / 256);
   V* yb = (V*)h->yb;
   for (size_t l = 0; l < h->lev.size(); ++l) {
@@ -863,32 +956,41 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
       // the level-0 couplings are sparse: y_e = Y_e r_e, then r_k -= L_k y_e1 + U_k y_e2
       // (one Y block per eliminated line instead of two multipliers per kept line)
       const int G = group_for(L.ne);
-      hipLaunchKernelGGL((k_bwd_x<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)rw, yb,
-                         cs);
+      with_rows(rows_for<B>(L.ne, m), [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        hipLaunchKernelGGL((k_bwd_x<B, V, NC, R>), dim3(gx(G, R), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L),
+                           (const V*)rw, yb, cs);
+      });
       hipLaunchKernelGGL((k_fwd_sparse<V, NC>), dim3(gs, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr, h->nLr, h->Ur,
                          h->nUr, (const V*)yb, rw, cs);
     } else {
       const int G = group_for(L.nk);
-      hipLaunchKernelGGL((k_fwd<B, V, NC>), dim3(gx(G), L.nk), blk, 0, st, m, G, vec, L.k, L.k1, L.k2, P1(L), P2(L),
-                         rw, cs);
+      with_rows(rows_for<B>(L.nk, m), [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        hipLaunchKernelGGL((k_fwd<B, V, NC, R>), dim3(gx(G, R), L.nk), blk, 0, st, m, G, vec, L.k, L.k1, L.k2, P1(L),
+                           P2(L), rw, cs);
+      });
     }
   }
   for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
-    if (l > 0 && h->fused) {
-      hipLaunchKernelGGL((k_bwd_fused<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Yb(L),
-                         Lb(L), Ub(L), (const V*)rw, xo, cs);
-      continue;
-    }
-    if (l == 0)
-      hipLaunchKernelGGL((k_bwd_t_sparse<V, NC>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr, h->Ur,
-                         h->nUr, (const V*)rw, (const V*)xo, tb, cs);
-    else
-      hipLaunchKernelGGL((k_bwd_t<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Lb(L), Ub(L),
-                         (const V*)rw, (const V*)xo, tb, cs);
-    hipLaunchKernelGGL((k_bwd_x<B, V, NC>), dim3(gx(G), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L), (const V*)tb, xo,
-                       cs);
+    with_rows(rows_for<B>(L.ne, m), [&](auto Rc) {
+      constexpr int R = decltype(Rc)::value;
+      if (l > 0 && h->fused) {
+        hipLaunchKernelGGL((k_bwd_fused<B, V, NC, R>), dim3(gx(G, R), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb,
+                           Yb(L), Lb(L), Ub(L), (const V*)rw, xo, cs);
+        return;
+      }
+      if (l == 0)
+        hipLaunchKernelGGL((k_bwd_t_sparse<V, NC>), dim3(gs, L.ne), blk, 0, st, m, L.e, L.ea, L.eb, h->Lr, h->nLr,
+                           h->Ur, h->nUr, (const V*)rw, (const V*)xo, tb, cs);
+      else
+        hipLaunchKernelGGL((k_bwd_t<B, V, NC, R>), dim3(gx(G, R), L.ne), blk, 0, st, m, G, vec, L.e, L.ea, L.eb, Lb(L),
+                           Ub(L), (const V*)rw, (const V*)xo, tb, cs);
+      hipLaunchKernelGGL((k_bwd_x<B, V, NC, R>), dim3(gx(G, R), L.ne), blk, 0, st, m, G, vec, L.e, Yb(L),
+                         (const V*)tb, xo, cs);
+    });
   }
 }
 
@@ -922,10 +1024,14 @@ void levels_half(tf_bcr* h, float* rw, float* tb, float* xo, long cs, hipStream_
       [](const BcrLevelDev& L) { return B{(f16* const*)L.Yh, (float* const*)L.Ys}; }, rw, tb, xo, cs, st);
 }
 
-template <typename T, typename V>
+// algorithmic bytes of an NC-column apply: the factor blocks once, six vectors per column
+template <typename T, typename V, int NC>
 double solve_work(const tf_bcr* h) {
-  return (double)((sizeof(T) == 4 && sizeof(V) == 4 && h->half) ? h->solve_bytes_half : h->solve_bytes);
+  const double one = (double)((sizeof(T) == 4 && sizeof(V) == 4 && h->half) ? h->solve_bytes_half : h->solve_bytes);
+  return one + (NC - 1) * 6.0 * sizeof(V) * (double)h->n;
 }
+
+constexpr int solve_prof_id(int nc) { return nc == 1 ? tf::PROF_SOLVE : tf::PROF_SOLVE2; }
 
 // NC right-hand sides r[j] -> solutions x[j] (host arrays of device pointers).  The
 // work vectors hold the NC columns at stride n.  prof: time the apply with the
@@ -936,8 +1042,8 @@ int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_
   V* rw = (V*)h->rw;
   V* tb = (V*)h->tb;
   if (prof) {
-    TF_PROF_WORK(tf::PROF_SOLVE, (solve_work<T, V>(h)));
-    TF_PROF_BEGIN(tf::PROF_SOLVE, st);
+    TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
+    TF_PROF_BEGIN(solve_prof_id(NC), st);
   }
   const bool nat = !h->order.ylines;
   // one column in natural order: the kernels write the solution in place
@@ -963,7 +1069,7 @@ int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_
                            x[j]);
     }
   }
-  if (prof) TF_PROF_END(tf::PROF_SOLVE, st);
+  if (prof) TF_PROF_END(solve_prof_id(NC), st);
   TF_LAUNCH_CHECK("bcr solve");
   return 0;
 }
@@ -1006,13 +1112,13 @@ int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream
     gs.exec = ex;
   }
   // the profiler's events bracket what an apply costs on this path: the staging copies and the replay
-  TF_PROF_WORK(tf::PROF_SOLVE, (solve_work<T, V>(h)));
-  TF_PROF_BEGIN(tf::PROF_SOLVE, st);
+  TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
+  TF_PROF_BEGIN(solve_prof_id(NC), st);
   for (int j = 0; j < NC; ++j) hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, r[j], gin + j * n);
   TF_HIP(hipGraphLaunch(gs.exec, st));
   for (int j = 0; j < NC; ++j)
     hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, (const V*)(gout + j * n), x[j]);
-  TF_PROF_END(tf::PROF_SOLVE, st);
+  TF_PROF_END(solve_prof_id(NC), st);
   TF_LAUNCH_CHECK("bcr graph apply");
   return 0;
 }

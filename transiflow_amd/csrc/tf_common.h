@@ -42,6 +42,7 @@ enum ProfId : int {
   PROF_SPMV = 7,
   PROF_CONV = 8,
   PROF_SOLVE = 9,
+  PROF_SOLVE2 = 10,  // two-column line-factor apply
   PROF_IDS = 16
 };
 bool prof_on();
