@@ -660,19 +660,30 @@ __global__ __launch_bounds__(256) void k_to_half(int m, float* const* __restrict
   if (lane == 0) scl[b][r] = ldexpf(1.0f, e - 15);
 }
 
-template <typename V>
-__global__ __launch_bounds__(256) void k_copy(long n, const V* __restrict__ a, V* __restrict__ b) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) b[e] = a[e];
+// NC column pointers passed by value
+template <typename P, int NC>
+struct Cols {
+  P p[NC];
+};
+
+// rw (line-major, NC columns at stride cs) <- the right-hand sides r[j] in natural order
+template <typename V, int NC>
+__global__ __launch_bounds__(256) void k_stage_in(int n, Order o, Cols<const V*, NC> r, V* __restrict__ rw, long cs) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int s = o.to_natural(q);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) rw[j * cs + q] = r.p[j][s];
+  }
 }
 
-// line-major <- natural (gather) and natural <- line-major (scatter)
-template <typename V>
-__global__ __launch_bounds__(256) void k_to_lines(int n, Order o, const V* __restrict__ nat, V* __restrict__ lin) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) lin[q] = nat[o.to_natural(q)];
-}
-template <typename V>
-__global__ __launch_bounds__(256) void k_to_natural(int n, Order o, const V* __restrict__ lin, V* __restrict__ nat) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) nat[o.to_natural(q)] = lin[q];
+// the solutions x[j] in natural order <- xo (line-major, NC columns at stride cs)
+template <typename V, int NC>
+__global__ __launch_bounds__(256) void k_stage_out(int n, Order o, const V* __restrict__ xo, long cs, Cols<V*, NC> x) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int s = o.to_natural(q);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x.p[j][s] = xo[j * cs + q];
+  }
 }
 
 // ---- host-side schedule ----------------------------------------------------------
@@ -753,10 +764,8 @@ struct tf_bcr {
   int n, m, N, single;
   Order order;
   void* xw;    // 2n (line-major solution when the order is not natural; two columns)
-  // captured apply: gin -> gout replayed as one graph launch (per vector type)
+  // captured apply: the sweeps rw -> xw replayed as one graph launch (per vector type and column count)
   int use_graph;
-  void* gin;   // 2n
-  void* gout;  // 2n
   hipStream_t cap;
   BcrGraph g64[2], g32[2];   // [columns - 1]
   size_t es;
@@ -1033,73 +1042,74 @@ double solve_work(const tf_bcr* h) {
 
 constexpr int solve_prof_id(int nc) { return nc == 1 ? tf::PROF_SOLVE : tf::PROF_SOLVE2; }
 
-// NC right-hand sides r[j] -> solutions x[j] (host arrays of device pointers).  The
-// work vectors hold the NC columns at stride n.  prof: time the apply with the
-// library profiler's events (never inside a capture).
+// The sweeps of an apply on the work vectors: rw (right-hand sides, line-major,
+// overwritten) -> xo (solutions, line-major), NC columns at stride n.
 template <typename T, typename V, int NC>
-int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st, bool prof = true) {
+void run_levels(tf_bcr* h, V* rw, V* xo, hipStream_t st) {
   const long n = h->n;
-  V* rw = (V*)h->rw;
   V* tb = (V*)h->tb;
-  if (prof) {
-    TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
-    TF_PROF_BEGIN(solve_prof_id(NC), st);
-  }
-  const bool nat = !h->order.ylines;
-  // one column in natural order: the kernels write the solution in place
-  V* xo = (nat && NC == 1) ? x[0] : (V*)h->xw;
-  for (int j = 0; j < NC; ++j) {
-    if (nat)
-      hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, r[j], rw + j * n);
-    else
-      hipLaunchKernelGGL(k_to_lines<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, r[j], rw + j * n);
-  }
   bool half = false;
   if constexpr (sizeof(T) == 4 && sizeof(V) == 4) half = h->half != 0;
   if constexpr (sizeof(T) == 4 && sizeof(V) == 4) {
     if (half) levels_half<NC>(h, (float*)rw, (float*)tb, (float*)xo, n, st);
   }
   if (!half) levels<T, V, NC>(h, rw, tb, xo, n, st);
-  if (xo != x[0] || NC > 1) {
-    for (int j = 0; j < NC; ++j) {
-      if (nat)
-        hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, (const V*)(xo + j * n), x[j]);
-      else
-        hipLaunchKernelGGL(k_to_natural<V>, dim3(1024), dim3(256), 0, st, h->n, h->order, (const V*)(xo + j * n),
-                           x[j]);
-    }
-  }
-  if (prof) TF_PROF_END(solve_prof_id(NC), st);
+}
+
+// r[j] (natural order) -> the work vector rw, one launch for all columns
+template <typename V, int NC>
+void stage_in(const tf_bcr* h, const V* const (&r)[NC], V* rw, hipStream_t st) {
+  Cols<const V*, NC> c;
+  for (int j = 0; j < NC; ++j) c.p[j] = r[j];
+  hipLaunchKernelGGL((k_stage_in<V, NC>), dim3(1024), dim3(256), 0, st, h->n, h->order, c, rw, (long)h->n);
+}
+
+// xo (line-major) -> x[j] (natural order), one launch for all columns
+template <typename V, int NC>
+void stage_out(const tf_bcr* h, const V* xo, V* const (&x)[NC], hipStream_t st) {
+  Cols<V*, NC> c;
+  for (int j = 0; j < NC; ++j) c.p[j] = x[j];
+  hipLaunchKernelGGL((k_stage_out<V, NC>), dim3(1024), dim3(256), 0, st, h->n, h->order, xo, (long)h->n, c);
+}
+
+// NC right-hand sides r[j] -> solutions x[j] (host arrays of device pointers), eager.
+template <typename T, typename V, int NC>
+int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st) {
+  V* rw = (V*)h->rw;
+  TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
+  TF_PROF_BEGIN(solve_prof_id(NC), st);
+  // one column in natural order: the kernels write the solution in place
+  const bool direct = !h->order.ylines && NC == 1;
+  V* xo = direct ? x[0] : (V*)h->xw;
+  stage_in<V, NC>(h, r, rw, st);
+  run_levels<T, V, NC>(h, rw, xo, st);
+  if (!direct) stage_out<V, NC>(h, xo, x, st);
+  TF_PROF_END(solve_prof_id(NC), st);
   TF_LAUNCH_CHECK("bcr solve");
   return 0;
 }
 
-// The apply captured once per (vector type, column count) on a private stream
-// (gin -> gout) and replayed with one graph launch; the factor refactors in
-// place, so the captured pointers stay valid.  The graph holds kernel nodes only.
+// The sweeps (rw -> xw) captured once per (vector type, column count) on a private
+// stream and replayed with one graph launch between the two staging kernels; the
+// factor refactors in place, so the captured pointers stay valid.  The graph holds
+// kernel nodes only.
 template <typename T, typename V, int NC>
 int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st) {
   BcrGraph& gs = (sizeof(V) == 8 ? h->g64 : h->g32)[NC - 1];
-  const long n = h->n;
-  V* gin = (V*)h->gin;
-  V* gout = (V*)h->gout;
+  V* rw = (V*)h->rw;
+  V* xw = (V*)h->xw;
   if (!gs.exec) {
     // non-blocking: a capture must not order against the legacy stream that other
     // threads use for synchronous copies
     if (!h->cap) TF_HIP(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
-    const V* ri[NC];
-    V* xi[NC];
-    for (int j = 0; j < NC; ++j) {
-      ri[j] = gin + j * n;
-      xi[j] = gout + j * n;
-    }
     TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-    const int e = solve_impl<T, V, NC>(h, ri, xi, h->cap, false);
+    run_levels<T, V, NC>(h, rw, xw, h->cap);
+    const hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(h->cap, &g);
-    if (e) {
+    if (le != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      return e;
+      return tf::fail_hip(le, "bcr apply capture");
     }
     TF_HIP(ce);
     hipGraphExec_t ex = nullptr;
@@ -1111,13 +1121,12 @@ int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream
     gs.graph = g;
     gs.exec = ex;
   }
-  // the profiler's events bracket what an apply costs on this path: the staging copies and the replay
+  // the profiler's events bracket what an apply costs on this path: the staging kernels and the replay
   TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
   TF_PROF_BEGIN(solve_prof_id(NC), st);
-  for (int j = 0; j < NC; ++j) hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, r[j], gin + j * n);
+  stage_in<V, NC>(h, r, rw, st);
   TF_HIP(hipGraphLaunch(gs.exec, st));
-  for (int j = 0; j < NC; ++j)
-    hipLaunchKernelGGL(k_copy<V>, dim3(1024), dim3(256), 0, st, n, (const V*)(gout + j * n), x[j]);
+  stage_out<V, NC>(h, xw, x, st);
   TF_PROF_END(solve_prof_id(NC), st);
   TF_LAUNCH_CHECK("bcr graph apply");
   return 0;
@@ -1217,8 +1226,6 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   TF_HIP(hipMalloc(&h->tb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->xw, 2 * sizeof(double) * (size_t)n));   // two columns
-  TF_HIP(hipMalloc(&h->gin, 2 * sizeof(double) * (size_t)n));   // two columns
-  TF_HIP(hipMalloc(&h->gout, 2 * sizeof(double) * (size_t)n));   // two columns
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -1457,8 +1464,6 @@ int tf_bcr_destroy(tf_bcr* h) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
   }
   if (h->cap) (void)hipStreamDestroy(h->cap);
-  (void)hipFree(h->gin);
-  (void)hipFree(h->gout);
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
                   (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W})
