@@ -22,7 +22,10 @@ preconditioner; the network is trained on the GPU before the timed runs on
 triples recorded from a 48 x 48 cavity continuation, and the correction
 judges itself and switches off when it stops paying).  ``value`` is the
 learned16 rate when it reproduces the FP64 variant's final state to 1e-8 (the
-metric's "learned+mixed"), else the fastest variant that does.
+metric's "learned+mixed"), else the fastest variant that does.  Each variant
+is timed ``--repeats`` times (3) and its run of median time is reported
+(run-to-run spread of a 0.5 s continuation is several percent); every run's
+rate is listed under ``variants.<name>.steps_per_s_runs``.
 
 The roofline object describes the kernel with the largest summed time in a
 profiled repeat of the headline variant (HIP events around each launch of the
@@ -33,6 +36,7 @@ file names the kernel.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import gc
 import ctypes
 import json
 import os
@@ -512,6 +516,8 @@ def main():
     ap.add_argument('--variants', default=None,
                     help="comma-separated solver variants (default: 'fp64,fp64reuse,mixed16,learned16' for the 2D "
                          "cavity, 'fp64,mixed16,learned16' for the 3D cavities, 'fp64,mixed16' for the others)")
+    ap.add_argument('--repeats', type=int, default=3,
+                    help='timed runs per variant; the run of median time is reported')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-hbm-sizes', action='store_true',
                     help='skip the assembly and SpMV rates at 1024^2 and 64^3 (working sets beyond the cache)')
@@ -564,11 +570,21 @@ def main():
 
     results = {}
     for var in args.variants.split(','):
-        log('timed run: %s' % var)
-        results[var] = run_continuation(tf, torch, spec, var, maxit)
-        r = results[var]
-        log('  %s: %.2f s, %d accepted steps, %.3f steps/s, %d factorizations, %d GMRES its'
-            % (var, r['seconds'], r['accepted'], r['steps_per_s'], r['factorizations'], r['gmres_iterations']))
+        runs = []
+        for rep in range(max(args.repeats, 1)):
+            log('timed run: %s (%d of %d)' % (var, rep + 1, max(args.repeats, 1)))
+            r = run_continuation(tf, torch, spec, var, maxit)
+            log('  %s: %.2f s, %d accepted steps, %.3f steps/s, %d factorizations, %d GMRES its'
+                % (var, r['seconds'], r['accepted'], r['steps_per_s'], r['factorizations'], r['gmres_iterations']))
+            runs.append(r)
+            gc.collect()
+        # the run of median time stands for the variant (the others are listed by rate)
+        runs.sort(key=lambda q: q['seconds'])
+        med = runs[len(runs) // 2]
+        med['steps_per_s_runs'] = [q['steps_per_s'] for q in runs]
+        results[var] = med
+        del runs, r
+        gc.collect()
 
     base = results.get('fp64')
     xref = base['x'].numpy() if base else None
@@ -653,7 +669,8 @@ def main():
             'variant': head,
             'parallelism': 'single GPU',
         },
-        'variants': {v: {k: r[k] for k in ('steps_per_s', 'seconds', 'accepted', 'corrector_iterations',
+        'variants': {v: {k: r[k] for k in ('steps_per_s', 'steps_per_s_runs', 'seconds', 'accepted',
+                                           'corrector_iterations',
                                            'factorizations', 'solves', 'gmres_iterations',
                                            'worst_relative_residual', 'background_factorizations', 'learned',
                                            'preconditioner', 'device_memory_GB')}

@@ -98,6 +98,21 @@ int tf_spmv_f32(int n, const int* row_ptr, const int* col, const float* val, con
 /* x[i] = v for 0 <= i < n (an error otherwise). */
 int tf_set_entry_f64(double* x, long n, long i, double v, void* stream);
 
+/* One round of the FP64 iterative refinement around an FP32 inner solve (the
+ * mixed-precision variant of SciPy.Interface.solve, SciPy.py:343-370), before the
+ * inner solve: r = b - A x (FP64, gauged as tf_spmv_f64), |r| into norm_dev (as
+ * tf_dot_f64 with norm = 1, work of tf_reduce_work_size() doubles), r32 = (float) r
+ * and d32 = 0.  The kernels of those calls one by one, in one call; nothing is
+ * synchronised. */
+int tf_refine_residual(int n, const int* row_ptr, const int* col, const double* val, int gauge, const double* x,
+                       const double* b, double* r, float* r32, float* d32, double* work, double* norm_dev,
+                       void* stream);
+/* After the inner solve: x += (double) d32, through d = (double) d32 (tf_f32_to_f64
+ * then tf_axpby_f64 in one call). */
+int tf_refine_update(long n, const float* d32, double* d, double* x, void* stream);
+/* count doubles from device memory to host memory; synchronises the stream. */
+int tf_read_f64(const double* dev, int count, double* host, void* stream);
+
 /* C = alpha A + beta B for CSR matrices (scipy sparse addition, as used by
  * TimeIntegration._newton, TimeIntegration.py:56: jacobian(x) - mass / (theta dt)).
  * A's entries keep their order; B's columns missing in A's row are appended. */

@@ -43,6 +43,9 @@ SIGNATURES = {
     'tf_reduce_work_size': (_c_int, []),
     'tf_dot_f64': (_c_int, [_c_long, _vp, _vp, _vp, _vp, _dp, _c_int, _vp]),
     'tf_spmv_f64': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
+    'tf_refine_residual': (_c_int, [_c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'tf_refine_update': (_c_int, [_c_long, _vp, _vp, _vp, _vp]),
+    'tf_read_f64': (_c_int, [_vp, _c_int, _dp, _vp]),
     'tf_spmv_f32': (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _vp]),
     'tf_set_entry_f64': (_c_int, [_vp, _c_long, _c_long, _c_double, _vp]),
     'tf_fill_f64': (_c_int, [_c_long, _c_double, _vp, _vp]),

@@ -560,6 +560,32 @@ int tf_spmv_f64(int n, const int* rp, const int* ci, const double* v, const doub
   return 0;
 }
 
+int tf_refine_residual(int n, const int* rp, const int* ci, const double* val, int gauge, const double* x,
+                       const double* b, double* r, float* r32, float* d32, double* work, double* norm_dev,
+                       void* stream) {
+  if (n < 1 || !rp || !ci || !val || !x || !b || !r || !r32 || !d32 || !work || !norm_dev)
+    return tf::fail("tf_refine_residual: bad arguments");
+  if (int e = tf_spmv_f64(n, rp, ci, val, x, r, gauge, stream)) return e;
+  if (int e = tf_axpby_f64(n, 1.0, b, -1.0, r, r, stream)) return e;
+  if (int e = tf_dot_f64(n, r, r, work, norm_dev, nullptr, 1, stream)) return e;
+  if (int e = tf_f64_to_f32(n, r, r32, stream)) return e;
+  return tf_fill_f32(n, 0.0, d32, stream);
+}
+
+int tf_refine_update(long n, const float* d32, double* d, double* x, void* stream) {
+  if (n < 1 || !d32 || !d || !x) return tf::fail("tf_refine_update: bad arguments");
+  if (int e = tf_f32_to_f64(n, d32, d, stream)) return e;
+  return tf_axpby_f64(n, 1.0, x, 1.0, d, x, stream);
+}
+
+int tf_read_f64(const double* dev, int count, double* host, void* stream) {
+  if (count < 0 || (count > 0 && (!dev || !host))) return tf::fail("tf_read_f64: bad arguments");
+  if (count == 0) return 0;
+  TF_HIP(hipMemcpyAsync(host, dev, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  TF_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
 int tf_spmv_f32(int n, const int* rp, const int* ci, const float* v, const float* x, float* y, int gauge,
                 void* stream) {
   hipLaunchKernelGGL((k_spmv<float, float>), dim3(tf::spmv_blocks<float>(n)), dim3(256), 0, (hipStream_t)stream, n, rp, ci, v,

@@ -14,8 +14,10 @@
 // column to pinned host memory) is captured once per j into a hipGraph and
 // replayed with one launch; the SpMV reads the matrix through a small device
 // descriptor that each solve points at its own CSR arrays, so the graphs stay
-// valid across Jacobians.  The host then waits for the column by polling the
-// stream (no blocking wait, whose wake-up would leave the GPU idle longer).
+// valid across Jacobians.  The host then waits for the column by polling (no
+// blocking wait, whose wake-up would leave the GPU idle longer); while the
+// residual estimate is far from the tolerance, the preconditioner on the new basis
+// vector is queued before that wait, so the device is busy through the host's part.
 #include "spmv.h"
 #include "tf_common.h"
 #include "transiflow_hip.h"
@@ -54,6 +56,7 @@ struct tf_krylov {
   hipGraph_t step2_graph[MAX_STEPS];
   unsigned long long uid;            // unique per handle (a freed handle's address can come back)
   unsigned long long step2_partner;  // uid of the partner the step2 graphs were captured with
+  hipEvent_t step_done;              // recorded after a step's Hessenberg column is queued to the host
   int use_graph;
 };
 
@@ -276,6 +279,15 @@ int poll_wait(hipStream_t st) {
   }
 }
 
+// Wait for event ev by polling.
+int poll_event(hipEvent_t ev) {
+  for (;;) {
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipSuccess) return 0;
+    if (q != hipErrorNotReady) return tf::fail_hip(q, "hipEventQuery");
+  }
+}
+
 // ---- restarted FGMRES, one or two columns ---------------------------------------
 
 // The state of one system's FGMRES (its Krylov handle, right-hand side, iterate,
@@ -375,6 +387,16 @@ void col_givens(Column& c, int j, int maxit) {
   if (std::fabs(c.g[j + 1]) / c.bnrm <= tol || c.its >= maxit || !std::isfinite(c.g[j + 1])) c.stepping = false;
 }
 
+// Whether a stepping column will most likely still be stepping after step j: the
+// iteration limit is not reached and its residual estimate |g_j| times the last
+// reduction factor |s_{j-1}| (the sine of the previous rotation) stays above four
+// times the tolerance.
+bool col_continues(const Column& c, int j, int maxit) {
+  if (j < 1 || c.its + 1 >= maxit) return false;
+  const double pred = std::fabs(c.g[j]) * std::fabs(c.sn[j - 1]);
+  return std::isfinite(pred) && pred > 4.0 * c.tol * c.bnrm;
+}
+
 // x += Z y with H y = g (upper triangular); a single-precision solve that ran out
 // of iterations ends on its residual estimate.
 int col_end_cycle(Column& c, int maxit, hipStream_t st) {
@@ -442,11 +464,14 @@ int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* 
       any = any || c[q].in_cycle;
     }
     if (!any) break;
+    // Z[j] of the stepping columns already queued by the previous step's look-ahead
+    bool have_z = false;
     for (int j = 0; j < m; ++j) {
       bool stepping = false;
       for (int q = 0; q < nc; ++q) stepping = stepping || c[q].stepping;
       if (!stepping) break;
-      if (int e = precond_columns(c, nc, j, prec_kind, prec, ncell, dof, st)) return e;
+      if (!have_z)
+        if (int e = precond_columns(c, nc, j, prec_kind, prec, ncell, dof, st)) return e;
       if (nc == 2 && c[0].stepping && c[1].stepping) {
         tf_krylov* a = c[0].k;
         tf_krylov* b = c[1].k;
@@ -460,7 +485,23 @@ int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* 
             return e;
         }
       }
-      if (int e = poll_wait(st)) return e;
+      // Look-ahead: when every stepping column will most likely go on, the
+      // preconditioner on V[j+1] (written by the step just queued) is queued before
+      // the host waits for the Hessenberg column, so the device works through the
+      // host's Givens update.  The arithmetic is unchanged; a wrong guess costs one
+      // unused Z[j+1].
+      bool ahead = j + 1 < m;
+      for (int q = 0; q < nc; ++q)
+        if (c[q].stepping && !col_continues(c[q], j, maxit)) ahead = false;
+      have_z = ahead;
+      if (ahead) {
+        hipEvent_t ev = c[0].k->step_done;
+        TF_HIP(hipEventRecord(ev, st));
+        if (int e = precond_columns(c, nc, j + 1, prec_kind, prec, ncell, dof, st)) return e;
+        if (int e = poll_event(ev)) return e;
+      } else if (int e = poll_wait(st)) {
+        return e;
+      }
       for (int q = 0; q < nc; ++q)
         if (c[q].stepping) col_givens(c[q], j, maxit);
     }
@@ -495,6 +536,7 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
     TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
     TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
     TF_HIP(hipMalloc(&k->desc, sizeof(MatrixDesc)));
+    TF_HIP(hipEventCreateWithFlags(&k->step_done, hipEventDisableTiming));
     return 0;
   };
   k->use_graph = 1;
@@ -523,6 +565,7 @@ int tf_krylov_destroy(tf_krylov* k) {
     if (k->step2_graph[j]) (void)hipGraphDestroy(k->step2_graph[j]);
   }
   if (k->cap) (void)hipStreamDestroy(k->cap);
+  if (k->step_done) (void)hipEventDestroy(k->step_done);
   for (void* p : {k->V, k->Z, k->w, (void*)k->h, (void*)k->y, (void*)k->work, (void*)k->desc}) (void)hipFree(p);
   (void)hipHostFree(k->h_host);
   delete k;

@@ -1000,6 +1000,22 @@ class Solver:
                          T.empty(n, dtype=T.float32, device='cuda'), T.empty(n, dtype=T.float64, device='cuda'))
         return self._buf
 
+    def _refine_residuals(self, jac, g, items, work):
+        """|b - A x| (FP64) for up to two (x, b, buffers) items, with r32 = (float) r
+        and d32 = 0 queued for the inner solve (tf_refine_residual per item); the
+        norms come back with one synchronisation."""
+        T = torch()
+        n, st = self.itf.n, stream()
+        if getattr(self, '_normbuf', None) is None:
+            self._normbuf = T.empty(2, dtype=T.float64, device='cuda')
+        nd = self._normbuf.data_ptr()
+        for q, (x, b, (r, r32, d32, _)) in enumerate(items):
+            _lib.call('tf_refine_residual', n, ptr(jac.row_ptr), ptr(jac.col), ptr(jac.val), int(g), ptr(x), ptr(b),
+                      ptr(r), ptr(r32), ptr(d32), ptr(work), ctypes.c_void_p(nd + 8 * q), st)
+        out = (ctypes.c_double * len(items))()
+        _lib.call('tf_read_f64', ctypes.c_void_p(nd), len(items), out, st)
+        return list(out)
+
     def _solve_mixed(self, jac, b, x, opts):
         """FP64 iterative refinement: r = b - A x (FP64), A d = r by FP32 FGMRES
         with the FP32 factor, x += d (FP64), until ||r|| <= tol ||b|| in FP64."""
@@ -1043,9 +1059,8 @@ class Solver:
         inner_max = min(opts['Restart'], opts['Inner Iterations'])
         while outer < opts['Maximum Refinements'] and total < opts['Maximum Iterations']:
             if outer > 0:
-                jac.matvec(x, out=r, gauge=g)
-                _lib.call('tf_axpby_f64', n, 1.0, ptr(b), -1.0, ptr(r), ptr(r), st)
-                rel = _norm(r, work) / bnrm
+                # r = b - A x, r32 = (float) r, d32 = 0
+                rel = self._refine_residuals(jac, g, [(x, b, (r, r32, d32, d))], work)[0] / bnrm
                 if rel <= opts['Tolerance']:
                     break
                 if bg is not None and bg.poll(jac):
@@ -1064,10 +1079,9 @@ class Solver:
                     self.refinements.append(outer)
                     return total, rel, prec, True
                 prev = rel
-                _lib.call('tf_f64_to_f32', n, ptr(r), ptr(r32), st)
             else:
                 _lib.call('tf_f64_to_f32', n, ptr(b), ptr(r32), st)
-            _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
+                _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
             # the last refinement step: when the remaining reduction is within what an
             # FP32 inner solve reaches, ask for it (with margin) instead of 'Inner Tolerance'
             itol = opts['Inner Tolerance']
@@ -1084,8 +1098,7 @@ class Solver:
                                      x_is_zero=True)
             self.trace.append((its, est, rel))
             total += its
-            _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)
-            _lib.call('tf_axpby_f64', n, 1.0, ptr(x), 1.0, ptr(d), ptr(x), st)
+            _lib.call('tf_refine_update', n, ptr(d32), ptr(d), ptr(x), st)   # x += (double) d32
             outer += 1
             # the limit is for 'Inner Tolerance': a tighter last step may take
             # proportionally more iterations without the factor being staler.  A
@@ -1204,27 +1217,26 @@ class Solver:
             if not live:
                 break
             refactor = False
-            for c in live:
+            # r = b - A x, r32 = (float) r, d32 = 0 for the columns past their first round
+            need = [c for c in live if cols[c]['outer'] > 0]
+            norms = self._refine_residuals(jac, g, [(xs[c], bs[c], bufs[c]) for c in need], work) if need else []
+            for c, nrm in zip(need, norms):
                 C = cols[c]
-                r, r32, d32, d = bufs[c]
-                if C['outer'] > 0:
-                    jac.matvec(xs[c], out=r, gauge=g)
-                    _lib.call('tf_axpby_f64', n, 1.0, ptr(bs[c]), -1.0, ptr(r), ptr(r), st)
-                    C['rel'] = _norm(r, work) / C['bnrm']
-                    if C['rel'] <= tol:
-                        C['done'] = True
-                        continue
-                    if C['rel'] > 0.5 * C['prev'] and not jac.prec_exact:
-                        refactor = True     # the lagged factor no longer contracts
-                    elif C['rel'] > 0.5 * C['prev'] and jac.prec_exact:
-                        # stagnation with this Jacobian's own FP32 factor: finish in FP64
-                        its64, C['rel'], p64, _ = self._fp64_fallback(jac, bs[c], xs[c], opts)
-                        C['total'] += its64
-                        C['prec'] = p64
-                        self.fallbacks += 1
-                        C['done'] = True
-                        continue
-                    C['prev'] = C['rel']
+                C['rel'] = nrm / C['bnrm']
+                if C['rel'] <= tol:
+                    C['done'] = True
+                    continue
+                if C['rel'] > 0.5 * C['prev'] and not jac.prec_exact:
+                    refactor = True     # the lagged factor no longer contracts
+                elif C['rel'] > 0.5 * C['prev'] and jac.prec_exact:
+                    # stagnation with this Jacobian's own FP32 factor: finish in FP64
+                    its64, C['rel'], p64, _ = self._fp64_fallback(jac, bs[c], xs[c], opts)
+                    C['total'] += its64
+                    C['prec'] = p64
+                    self.fallbacks += 1
+                    C['done'] = True
+                    continue
+                C['prev'] = C['rel']
             live = [c for c in range(2) if not cols[c]['done']]
             if not live:
                 break
@@ -1239,8 +1251,9 @@ class Solver:
             for c in live:
                 C = cols[c]
                 r, r32, d32, d = bufs[c]
-                _lib.call('tf_f64_to_f32', n, ptr(r if C['outer'] > 0 else bs[c]), ptr(r32), st)
-                _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
+                if C['outer'] == 0:
+                    _lib.call('tf_f64_to_f32', n, ptr(bs[c]), ptr(r32), st)
+                    _lib.call('tf_fill_f32', n, 0.0, ptr(d32), st)
                 itol = itol0
                 if C['outer'] > 0:
                     need = 0.5 * tol / C['rel']
@@ -1262,8 +1275,7 @@ class Solver:
                 its, est = inner[c]
                 self.trace.append((its, est, C['rel']))
                 C['total'] += its
-                _lib.call('tf_f32_to_f64', n, ptr(d32), ptr(d), st)
-                _lib.call('tf_axpby_f64', n, 1.0, ptr(xs[c]), 1.0, ptr(d), ptr(xs[c]), st)
+                _lib.call('tf_refine_update', n, ptr(d32), ptr(d), ptr(xs[c]), st)   # x += (double) d32
                 C['outer'] += 1
                 if jac.prec_exact and C['outer'] == 1 and itols[c] == itol0:
                     fresh_its.append(its)
