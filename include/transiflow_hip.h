@@ -224,10 +224,18 @@ int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
  * Newton-corrector iteration (Continuation.py:158-159: F_x z1 = -F, F_x z2 = F_mu). */
 int tf_bcr_solve2_f64(tf_bcr* h, const double* r0, const double* r1, double* x0, double* x1, void* stream);
 int tf_bcr_solve2_f32(tf_bcr* h, const float* r0, const float* r1, float* x0, float* x1, void* stream);
-/* on != 0: the apply is captured once (per vector type) into a hipGraph and replayed with
- * one launch plus two staging copies (no reference counterpart; launch overhead on small
- * grids and batches). */
+/* on != 0: the level sweeps of the apply are captured once (per vector type and column
+ * count) into a hipGraph and replayed with one launch between two staging kernels (no
+ * reference counterpart; launch overhead on small grids and batches). */
 int tf_bcr_set_graph(tf_bcr* h, int on);
+/* on != 0: with the fused backward, the levels with fewer than 12288 rows per launch run
+ * as one ordered-task launch instead of one launch per level (no reference counterpart;
+ * the launch floor of the upper levels).  Off by default and not in use: its first GPU
+ * run did not finish in time and the cause is not established.
+ * tf_bcr_top_errors: waits of that launch that gave up since the last call (then reset);
+ * synchronises the stream. */
+int tf_bcr_set_top(tf_bcr* h, int on);
+int tf_bcr_top_errors(tf_bcr* h, int* count_host, void* stream);
 /* on != 0 (single != 0 only): the FP32 apply (tf_bcr_solve_f32) reads FP16 copies of the
  * factor blocks with one power-of-two FP32 scale per row, remade after every factorisation:
  * half the bytes of the HBM-bound apply for the inner solves of the mixed-precision solver

@@ -604,6 +604,138 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const 
         for (int j = 0; j < NC; ++j) x[j * cs + base + rr[i]] = (V)acc[i][j];
 }
 
+// ---- the upper levels in one launch ---------------------------------------------
+//
+// The levels with few rows (batch x m < TOP_ROWS: the last forward levels and the
+// first backward levels, one group row each, R = 1) take a few microseconds of
+// launch and dependent-load latency each while reading little.  k_top runs them
+// all in one launch as an ordered list of tasks: a task is one block of rows of
+// one line at one level step (forward levels upward, then the fused backward
+// levels downward), handed out by an atomic ticket in that order.  A task waits
+// until every task of the previous step has finished (a per-step completion
+// counter), so a block only ever waits for tasks with smaller tickets, which
+// blocks that are already running hold: the launch makes progress whatever
+// number of its blocks is resident.  Each row's arithmetic is that of k_fwd /
+// k_bwd_fused, so the results are those of the per-level launches bit for bit.
+// Waits are bounded (TOP_SPIN polls); a wait that runs out sets the error word
+// and the task goes on.  The last block to leave resets the counters for the
+// next launch.  Applies of one handle must not overlap (as for its work vectors).
+constexpr int TOP_ROWS = 12288;
+constexpr int MAX_TOP = 16;          // steps of one launch
+constexpr long TOP_SPIN = 1L << 16;  // polls (about 1 us each) before a wait gives up
+// sync words: [0] ticket, [1] blocks finished, [2 .. 2+MAX_TOP) tasks done per step, [2+MAX_TOP] error
+constexpr int TOP_SYNC_WORDS = 3 + MAX_TOP;
+constexpr int TOP_BLOCKS = 512;        // blocks of a k_top launch
+
+template <typename B>
+struct TopStep {
+  int dir;    // 0: forward (kept lines, rw[k] -= P1 rw[e1] + P2 rw[e2]); 1: fused backward
+  int batch;  // lines
+  int G;      // lanes per row
+  int nrb;    // row blocks per line
+  int first;  // first ticket of the step
+  const int* line;
+  const int* n1;
+  const int* n2;
+  B blk[3];   // forward: P1, P2; backward: Y, Y L, Y U
+};
+
+template <typename B>
+struct TopPlan {
+  int nsteps, total;
+  TopStep<B> s[MAX_TOP];
+};
+
+template <typename B, typename V, int NC>
+__global__ __launch_bounds__(256) void k_top(int m, int vec, TopPlan<B> tp, V* __restrict__ rw, V* __restrict__ xo,
+                                             long cs, unsigned* __restrict__ sync) {
+  __shared__ int s_t;
+  for (;;) {
+    if (threadIdx.x == 0) s_t = (int)atomicAdd(&sync[0], 1u);
+    __syncthreads();
+    const int t = s_t;
+    __syncthreads();
+    if (t >= tp.total) break;
+    int si = 0;
+    while (si + 1 < tp.nsteps && t >= tp.s[si + 1].first) ++si;
+    const TopStep<B>& S = tp.s[si];
+    if (si > 0) {
+      if (threadIdx.x == 0) {
+        const unsigned need = (unsigned)(S.first - tp.s[si - 1].first);
+        long spins = 0;
+        while (__hip_atomic_load(&sync[2 + si - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (++spins > TOP_SPIN) {
+            atomicOr(&sync[2 + MAX_TOP], 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+      }
+      __syncthreads();
+    }
+    const int local = t - S.first;
+    const int b = local / S.nrb, rb = local - b * S.nrb;
+    const int G = S.G;
+    const int g = threadIdx.x % G;
+    const int r = rb * rows_per_block(G) + (int)threadIdx.x / G;
+    const bool live = r < m;
+    const int rr[1] = {live ? r : 0};
+    double d[1][NC];
+    if (S.dir == 0) {
+      double acc[NC];
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+      const int e1 = S.n1[b], e2 = S.n2[b];
+      if (e1 >= 0) {
+        row_dot<NC, 1>(S.blk[0], b, rr, rw + (long)e1 * m, cs, m, g, G, vec, d);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j] += d[0][j];
+      }
+      if (e2 >= 0) {
+        row_dot<NC, 1>(S.blk[1], b, rr, rw + (long)e2 * m, cs, m, g, G, vec, d);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[j] += d[0][j];
+      }
+      if (live && g == 0)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          V* out = rw + j * cs + (long)S.line[b] * m + r;
+          *out = (V)((double)*out - acc[j]);
+        }
+    } else {
+      const long base = (long)S.line[b] * m;
+      const int a = S.n1[b], bb = S.n2[b];
+      double acc[1][NC];
+      row_dot<NC, 1>(S.blk[0], b, rr, rw + base, cs, m, g, G, vec, acc);
+      if (a >= 0) {
+        row_dot<NC, 1>(S.blk[1], b, rr, xo + (long)a * m, cs, m, g, G, vec, d);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[0][j] -= d[0][j];
+      }
+      if (bb >= 0) {
+        row_dot<NC, 1>(S.blk[2], b, rr, xo + (long)bb * m, cs, m, g, G, vec, d);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[0][j] -= d[0][j];
+      }
+      if (live && g == 0)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) xo[j * cs + base + r] = (V)acc[0][j];
+    }
+    __syncthreads();  // the block's writes of this task are done
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(&sync[2 + si], 1u);
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (atomicAdd(&sync[1], 1u) == gridDim.x - 1) {
+      // every block has left: no task is waiting; reset for the next launch
+      for (int q = 0; q < 2 + MAX_TOP; ++q) __hip_atomic_store(&sync[q], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // copy a batch of m x m blocks: dst[b] = src[b]   (grid.y = batch)
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_blocks(long mm, T* const* __restrict__ src, T* const* __restrict__ dst) {
@@ -801,6 +933,8 @@ struct tf_bcr {
   size_t solve_bytes_half;
   int split;           // level GEMMs of an FP32 factor on BF16 terms: 0 off, 1 or 3 terms
   int fused;           // L, U of the levels >= 1 hold Y L, Y U (one backward kernel per level)
+  int use_top;         // the upper levels of the fused backward and their forward levels in one k_top launch
+  unsigned* topsync;   // k_top's counters and error word (TOP_SYNC_WORDS)
   void* W;             // scratch of the fused products (largest level, m^2 each)
 };
 
@@ -958,7 +1092,47 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
   auto gx = [m](int G, int R) { return (unsigned)((m + R * rows_per_block(G) - 1) / (R * rows_per_block(G))); };
   const unsigned gs = (unsigned)((m + 255) / 256);
   V* yb = (V*)h->yb;
-  for (size_t l = 0; l < h->lev.size(); ++l) {
+  const int nlev = (int)h->lev.size();
+  // the upper levels (every level from lt up has fewer than TOP_ROWS rows per
+  // launch) go to one k_top launch when the backward is fused
+  int lt = nlev;
+  while (lt > 1 && (long)std::max(h->lev[lt - 1].nk, h->lev[lt - 1].ne) * m < TOP_ROWS) --lt;
+  auto steps_from = [&](int l0) {
+    int c = 0;
+    for (int l = l0; l < nlev; ++l) c += (h->lev[l].nk > 0) + (h->lev[l].ne > 0);
+    return c;
+  };
+  while (lt < nlev && steps_from(lt) > MAX_TOP) ++lt;
+  TopPlan<B> tp{};
+  if (h->fused && h->use_top && lt < nlev) {
+    auto add = [&](int dir, int batch, const int* line, const int* n1, const int* n2, B b0, B b1, B b2) {
+      TopStep<B>& S = tp.s[tp.nsteps++];
+      S.dir = dir;
+      S.batch = batch;
+      S.G = group_for(batch);
+      S.nrb = (m + rows_per_block(S.G) - 1) / rows_per_block(S.G);
+      S.first = tp.total;
+      S.line = line;
+      S.n1 = n1;
+      S.n2 = n2;
+      S.blk[0] = b0;
+      S.blk[1] = b1;
+      S.blk[2] = b2;
+      tp.total += batch * S.nrb;
+    };
+    for (int l = lt; l < nlev; ++l) {
+      const BcrLevelDev& L = h->lev[l];
+      if (L.nk) add(0, L.nk, L.k, L.k1, L.k2, P1(L), P2(L), P2(L));
+    }
+    for (int l = nlev - 1; l >= lt; --l) {
+      const BcrLevelDev& L = h->lev[l];
+      if (L.ne) add(1, L.ne, L.e, L.ea, L.eb, Yb(L), Lb(L), Ub(L));
+    }
+    if (tp.nsteps < 2) tp = TopPlan<B>{};
+  }
+  const bool top = tp.nsteps >= 2;
+  const int fwd_end = top ? lt : nlev;
+  for (int l = 0; l < fwd_end; ++l) {
     const BcrLevelDev& L = h->lev[l];
     if (!L.nk) continue;
     if (l == 0) {
@@ -981,7 +1155,9 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
       });
     }
   }
-  for (int l = (int)h->lev.size() - 1; l >= 0; --l) {
+  if (top)
+    hipLaunchKernelGGL((k_top<B, V, NC>), dim3(TOP_BLOCKS), blk, 0, st, m, vec, tp, rw, xo, cs, h->topsync);
+  for (int l = top ? lt - 1 : nlev - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
     with_rows(rows_for<B>(L.ne, m), [&](auto Rc) {
@@ -1226,6 +1402,9 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   TF_HIP(hipMalloc(&h->tb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->xw, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->topsync, sizeof(unsigned) * TOP_SYNC_WORDS));
+  TF_HIP(hipMemset(h->topsync, 0, sizeof(unsigned) * TOP_SYNC_WORDS));
+  h->use_top = 0;  // off: see tf_bcr_set_top
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -1466,7 +1645,8 @@ int tf_bcr_destroy(tf_bcr* h) {
   if (h->cap) (void)hipStreamDestroy(h->cap);
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
-                  (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W})
+                  (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W,
+                  (void*)h->topsync})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;
@@ -1489,6 +1669,41 @@ int tf_bcr_status(tf_bcr* h, int* flags_host, void* stream) {
   TF_HIP(hipStreamSynchronize(st));
   flags_host[0] = h->host_flags[0];
   flags_host[1] = h->host_flags[1];
+  return 0;
+}
+
+// on != 0: the upper levels of an apply with the fused backward run as one k_top
+// launch; 0 (default): one launch per level.  Off by default: its first run on
+// the GPU did not finish within the test's time limit (a 48 x 48 FP32 factor) and
+// the cause is not established; the per-level launches are unchanged.
+int tf_bcr_set_top(tf_bcr* h, int on) {
+  if (!h) return tf::fail("null handle");
+  if ((on != 0) != (h->use_top != 0)) {
+    for (BcrGraph* g : {&h->g64[0], &h->g64[1], &h->g32[0], &h->g32[1]}) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      *g = BcrGraph{};
+    }
+  }
+  h->use_top = on ? 1 : 0;
+  return 0;
+}
+
+// The number of k_top waits that gave up since the last call (then cleared);
+// synchronises the stream.  Non-zero means the upper levels of some apply read
+// vectors before the level below them had finished.
+int tf_bcr_top_errors(tf_bcr* h, int* count_host, void* stream) {
+  if (!h || !count_host) return tf::fail("null argument");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned v = 0;
+  TF_HIP(hipMemcpyAsync(&v, h->topsync + 2 + MAX_TOP, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  TF_HIP(hipStreamSynchronize(st));
+  if (v) {
+    const unsigned zero = 0;
+    TF_HIP(hipMemcpyAsync(h->topsync + 2 + MAX_TOP, &zero, sizeof(unsigned), hipMemcpyHostToDevice, st));
+    TF_HIP(hipStreamSynchronize(st));
+  }
+  *count_host = (int)v;
   return 0;
 }
 

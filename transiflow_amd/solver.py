@@ -242,6 +242,22 @@ class LineFactor:
             raise SolverError('line factorisation: matrix is not block tridiagonal over grid lines (flags %d)'
                               % flags[0])
         self.perturbed = flags[1]
+        self.check_applies(st)
+
+    def check_applies(self, st):
+        """Raise if a wait of the upper-level launch of an apply since the last
+        check gave up (synchronises ``st``)."""
+        if self._p == 'tf_bcr':
+            n = ctypes.c_int()
+            _lib.call('tf_bcr_top_errors', self._h, ctypes.byref(n), st)
+            if n.value:
+                raise SolverError('line-factor apply: %d upper-level waits gave up' % n.value)
+
+    def set_top(self, on):
+        """The upper levels of the apply in one launch (block cyclic reduction with
+        the fused backward); the results do not change."""
+        if self._p == 'tf_bcr':
+            _lib.call('tf_bcr_set_top', self._h, 1 if on else 0)
 
     def apply(self, r, out):
         _lib.call(self._p + '_solve_f64', self._h, ptr(r), ptr(out), stream())
