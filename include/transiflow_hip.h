@@ -235,6 +235,10 @@ int tf_bcr_set_graph(tf_bcr* h, int on);
  * tf_bcr_top_errors: waits of that launch that gave up since the last call (then reset);
  * synchronises the stream. */
 int tf_bcr_set_top(tf_bcr* h, int on);
+/* on != 0 (default): levels with at most 8 lines get their line indices and block row
+ * bases as kernel arguments instead of reading them from the schedule's device lists;
+ * results equal bit for bit (no reference counterpart; latency of the upper levels). */
+int tf_bcr_set_small(tf_bcr* h, int on);
 int tf_bcr_top_errors(tf_bcr* h, int* count_host, void* stream);
 /* on != 0 (single != 0 only): the FP32 apply (tf_bcr_solve_f32) reads FP16 copies of the
  * factor blocks with one power-of-two FP32 scale per row, remade after every factorisation:

@@ -103,6 +103,7 @@ SIGNATURES = {
     'tf_bcr_solve2_f32': (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'tf_bcr_set_graph': (_c_int, [_vp, _c_int]),
     'tf_bcr_set_top': (_c_int, [_vp, _c_int]),
+    'tf_bcr_set_small': (_c_int, [_vp, _c_int]),
     'tf_bcr_top_errors': (_c_int, [_vp, _ip, _vp]),
     'tf_bcr_set_half': (_c_int, [_vp, _c_int, _vp]),
     'tf_bcr_set_gemm_split': (_c_int, [_vp, _c_int]),

@@ -385,6 +385,28 @@ struct HalfBlocks {
   __device__ __forceinline__ double scale(int b, int r) const { return (double)s[b][r]; }
 };
 
+// One block's rows with its base pointers already resolved (the few-line levels
+// pass them by value, so no pointer list is read before the rows).
+template <typename B>
+struct RowOf;
+template <typename T>
+struct RowOf<Blocks<T>> {
+  struct type {
+    const T* p;
+    __device__ __forceinline__ const T* row(int r, int m) const { return p + (long)r * m; }
+    __device__ __forceinline__ double scale(int) const { return 1.0; }
+  };
+};
+template <>
+struct RowOf<HalfBlocks> {
+  struct type {
+    const f16* p;
+    const float* s;
+    __device__ __forceinline__ const f16* row(int r, int m) const { return p + (long)r * m; }
+    __device__ __forceinline__ double scale(int r) const { return (double)s[r]; }
+  };
+};
+
 // Rows per lane group R: with two, the rows share each load of the vector chunk.
 // Two for FP16 rows once a launch has 12288 rows (batch x m) or more, so that the
 // halved group count still fills the device; one for rows in the factor's own
@@ -602,6 +624,81 @@ __global__ __launch_bounds__(256) void k_bwd_fused(int m, int G, int vec, const 
       if (live[i])
 #pragma unroll
         for (int j = 0; j < NC; ++j) x[j * cs + base + rr[i]] = (V)acc[i][j];
+}
+
+// ---- the few-line levels with their operands by value ----------------------------
+//
+// A level with at most SMALL_MAX lines is latency-bound: its launch reads little,
+// and every load of a line index or a block pointer from the schedule's lists
+// delays the row loads behind it.  k_small takes the line indices and the blocks'
+// row bases as kernel arguments (resolved on the host from a copy of the lists),
+// so the row and vector loads issue at once.  dir 0: the forward step of k_fwd;
+// dir 1: the fused backward step of k_bwd_fused; the same arithmetic (R = 1).
+constexpr int SMALL_MAX = 8;
+
+template <typename B>
+struct SmallLevel {
+  int line[SMALL_MAX], n1[SMALL_MAX], n2[SMALL_MAX];
+  typename RowOf<B>::type blk[3][SMALL_MAX];  // forward: P1, P2; backward: Y, Y L, Y U
+};
+
+template <int NC, typename Row, typename V>
+__device__ __forceinline__ void small_dot(const Row& row, int r, const V* __restrict__ x, long cs, int m, int g,
+                                          int G, bool vec, double (&out)[NC]) {
+  decltype(row.row(0, m)) a[1] = {row.row(r, m)};
+  double o[1][NC];
+  group_dot<NC, 1>(a, x, cs, m, g, G, vec, o);
+  const double sc = row.scale(r);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) out[j] = o[0][j] * sc;
+}
+
+template <typename B, typename V, int NC>
+__global__ __launch_bounds__(256) void k_small(int m, int G, int vec, int dir, SmallLevel<B> a, V* __restrict__ rw,
+                                               V* __restrict__ xo, long cs) {
+  const int b = blockIdx.y;
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * rows_per_block(G) + (int)threadIdx.x / G;
+  const bool live = r < m;
+  const int rr = live ? r : 0;
+  const int n1 = a.n1[b], n2 = a.n2[b];
+  double d[NC], acc[NC];
+  if (dir == 0) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+    if (n1 >= 0) {
+      small_dot<NC>(a.blk[0][b], rr, rw + (long)n1 * m, cs, m, g, G, vec, d);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    }
+    if (n2 >= 0) {
+      small_dot<NC>(a.blk[1][b], rr, rw + (long)n2 * m, cs, m, g, G, vec, d);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    }
+    if (live && g == 0)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        V* out = rw + j * cs + (long)a.line[b] * m + r;
+        *out = (V)((double)*out - acc[j]);
+      }
+  } else {
+    const long base = (long)a.line[b] * m;
+    small_dot<NC>(a.blk[0][b], rr, rw + base, cs, m, g, G, vec, acc);
+    if (n1 >= 0) {
+      small_dot<NC>(a.blk[1][b], rr, xo + (long)n1 * m, cs, m, g, G, vec, d);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+    }
+    if (n2 >= 0) {
+      small_dot<NC>(a.blk[2][b], rr, xo + (long)n2 * m, cs, m, g, G, vec, d);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] -= d[j];
+    }
+    if (live && g == 0)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) xo[j * cs + base + r] = (V)acc[j];
+  }
 }
 
 // ---- the upper levels in one launch ---------------------------------------------
@@ -934,6 +1031,8 @@ struct tf_bcr {
   int split;           // level GEMMs of an FP32 factor on BF16 terms: 0 off, 1 or 3 terms
   int fused;           // L, U of the levels >= 1 hold Y L, Y U (one backward kernel per level)
   int use_top;         // the upper levels of the fused backward and their forward levels in one k_top launch
+  int use_small;       // levels with at most SMALL_MAX lines through k_small (operands by value)
+  std::vector<char> pool_host, hpool_host;  // host copies of pool and hpool (k_small's operands)
   unsigned* topsync;   // k_top's counters and error word (TOP_SYNC_WORDS)
   void* W;             // scratch of the fused products (largest level, m^2 each)
 };
@@ -1082,6 +1181,47 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
   return 0;
 }
 
+// Host value of entry i of a device list that lives in the handle's pool or hpool
+// (false when the entry is in neither).
+template <typename P>
+bool host_entry(const tf_bcr* h, const void* dev_list, int i, P& v) {
+  const char* d = (const char*)dev_list;
+  const size_t at = sizeof(P) * (size_t)i;
+  for (int q = 0; q < 2; ++q) {
+    const char* d0 = (const char*)(q == 0 ? h->pool : h->hpool);
+    const std::vector<char>& mirror = q == 0 ? h->pool_host : h->hpool_host;
+    if (d0 && d >= d0 && (size_t)(d - d0) + at + sizeof(P) <= mirror.size()) {
+      memcpy(&v, mirror.data() + (d - d0) + at, sizeof(P));
+      return true;
+    }
+  }
+  return false;
+}
+
+template <typename T>
+bool host_row(const tf_bcr* h, const Blocks<T>& b, int i, typename RowOf<Blocks<T>>::type& r) {
+  return host_entry(h, b.p, i, r.p);
+}
+inline bool host_row(const tf_bcr* h, const HalfBlocks& b, int i, RowOf<HalfBlocks>::type& r) {
+  return host_entry(h, b.p, i, r.p) && host_entry(h, b.s, i, r.s);
+}
+
+// k_small's operands of one level step (batch <= SMALL_MAX lines); false when a list
+// is not in the handle's pools (the caller then uses the list-reading kernels)
+template <typename B>
+bool small_level(const tf_bcr* h, int batch, const int* line, const int* n1, const int* n2, const B& b0,
+                 const B& b1, const B* b2, SmallLevel<B>& a) {
+  a = SmallLevel<B>{};
+  if (batch < 1 || batch > SMALL_MAX) return false;
+  for (int i = 0; i < batch; ++i) {
+    if (!host_entry(h, line, i, a.line[i]) || !host_entry(h, n1, i, a.n1[i]) || !host_entry(h, n2, i, a.n2[i]) ||
+        !host_row(h, b0, i, a.blk[0][i]) || !host_row(h, b1, i, a.blk[1][i]) ||
+        (b2 && !host_row(h, *b2, i, a.blk[2][i])))
+      return false;
+  }
+  return true;
+}
+
 // forward and backward sweeps over the levels, rows read through the block
 // source B, for NC columns at stride cs in rw, tb, xo (and the handle's yb)
 template <typename B, typename V, int NC>
@@ -1146,6 +1286,10 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
       });
       hipLaunchKernelGGL((k_fwd_sparse<V, NC>), dim3(gs, L.nk), blk, 0, st, m, L.k, L.k1, L.k2, h->Lr, h->nLr, h->Ur,
                          h->nUr, (const V*)yb, rw, cs);
+    } else if (SmallLevel<B> sa; h->use_small && L.nk <= SMALL_MAX && rows_for<B>(L.nk, m) == 1 &&
+                                 small_level<B>(h, L.nk, L.k, L.k1, L.k2, P1(L), P2(L), nullptr, sa)) {
+      hipLaunchKernelGGL((k_small<B, V, NC>), dim3(gx(group_for(L.nk), 1), L.nk), blk, 0, st, m, group_for(L.nk),
+                         vec, 0, sa, rw, xo, cs);
     } else {
       const int G = group_for(L.nk);
       with_rows(rows_for<B>(L.nk, m), [&](auto Rc) {
@@ -1160,6 +1304,13 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
   for (int l = top ? lt - 1 : nlev - 1; l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
+    if (SmallLevel<B> sa; l > 0 && h->fused && h->use_small && L.ne <= SMALL_MAX && rows_for<B>(L.ne, m) == 1) {
+      const B ub = Ub(L);
+      if (small_level<B>(h, L.ne, L.e, L.ea, L.eb, Yb(L), Lb(L), &ub, sa)) {
+        hipLaunchKernelGGL((k_small<B, V, NC>), dim3(gx(G, 1), L.ne), blk, 0, st, m, G, vec, 1, sa, rw, xo, cs);
+        continue;
+      }
+    }
     with_rows(rows_for<B>(L.ne, m), [&](auto Rc) {
       constexpr int R = decltype(Rc)::value;
       if (l > 0 && h->fused) {
@@ -1405,6 +1556,7 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   TF_HIP(hipMalloc(&h->topsync, sizeof(unsigned) * TOP_SYNC_WORDS));
   TF_HIP(hipMemset(h->topsync, 0, sizeof(unsigned) * TOP_SYNC_WORDS));
   h->use_top = 0;  // off: see tf_bcr_set_top
+  h->use_small = 1;
 
   // device pointer lists of the schedule
   char* D = (char*)h->D;
@@ -1579,6 +1731,7 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   pb.align();
   TF_HIP(hipMalloc(&h->pool, pb.bytes.size() + 64));
   TF_HIP(tf::upload(h->pool, pb.bytes.data(), pb.bytes.size()));
+  h->pool_host = pb.bytes;
   char* P = (char*)h->pool;
   for (const Off& o : offs) {
     BcrLevelDev L{};
@@ -1686,6 +1839,22 @@ int tf_bcr_set_top(tf_bcr* h, int on) {
     }
   }
   h->use_top = on ? 1 : 0;
+  return 0;
+}
+
+// on != 0 (default): levels with at most SMALL_MAX lines run k_small, with the line
+// indices and block row bases passed by value; 0: k_fwd / k_bwd_fused as for the
+// other levels.  The results are the same bit for bit.
+int tf_bcr_set_small(tf_bcr* h, int on) {
+  if (!h) return tf::fail("null handle");
+  if ((on != 0) != (h->use_small != 0)) {
+    for (BcrGraph* g : {&h->g64[0], &h->g64[1], &h->g32[0], &h->g32[1]}) {
+      if (g->exec) (void)hipGraphExecDestroy(g->exec);
+      if (g->graph) (void)hipGraphDestroy(g->graph);
+      *g = BcrGraph{};
+    }
+  }
+  h->use_small = on ? 1 : 0;
   return 0;
 }
 
@@ -1811,6 +1980,7 @@ int tf_bcr_set_half(tf_bcr* h, int on, void* stream) {
     h->H = H_;
     h->S = S_;
     h->hpool = pool;
+    h->hpool_host = pb.bytes;
     char* P = (char*)h->hpool;
     for (size_t l = 0; l < offs.size(); ++l) {
       BcrLevelDev& L = h->lev[l];

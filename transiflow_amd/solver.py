@@ -253,6 +253,12 @@ class LineFactor:
             if n.value:
                 raise SolverError('line-factor apply: %d upper-level waits gave up' % n.value)
 
+    def set_small(self, on):
+        """Levels with at most 8 lines get their operands as kernel arguments (block
+        cyclic reduction); the results do not change."""
+        if self._p == 'tf_bcr':
+            _lib.call('tf_bcr_set_small', self._h, 1 if on else 0)
+
     def set_top(self, on):
         """The upper levels of the apply in one launch (block cyclic reduction with
         the fused backward); the results do not change."""
