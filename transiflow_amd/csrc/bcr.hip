@@ -541,7 +541,9 @@ __global__ __launch_bounds__(256) void k_bwd_t(int m, int G, int vec, const int*
         }
 }
 
-// Level 0: t[e] = rw[e] - L_e x[a] - U_e x[b] with the sparse couplings.  Thread per row.
+// Level 0: t[e] = rw[e] - L_e x[a] - U_e x[b] with the sparse couplings.  Thread per
+// row; the row's couplings are loaded once for all columns, each column's sum keeps
+// the coupling order.
 template <typename V, int NC>
 __global__ __launch_bounds__(256) void k_bwd_t_sparse(int m, const int* __restrict__ e_line,
                                                       const int* __restrict__ a_line, const int* __restrict__ b_line,
@@ -554,15 +556,23 @@ __global__ __launch_bounds__(256) void k_bwd_t_sparse(int m, const int* __restri
   if (r >= m) return;
   const long row = (long)e_line[bi] * m + r;
   const int a = a_line[bi], bb = b_line[bi];
+  const int nl = a >= 0 ? nLr[row] : 0, nu = bb >= 0 ? nUr[row] : 0;
+  Cpl lc[KC], uc[KC];
+#pragma unroll
+  for (int q = 0; q < KC; ++q) {
+    if (q < nl) lc[q] = Lr[row * KC + q];
+    if (q < nu) uc[q] = Ur[row * KC + q];
+  }
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     const V* xj = x + j * cs;
     double acc = 0.0;
-    if (a >= 0)
-      for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)xj[(long)a * m + Lr[row * KC + q].idx];
-    if (bb >= 0)
-      for (int q = 0; q < nUr[row]; ++q)
-        acc += Ur[row * KC + q].val * (double)xj[(long)bb * m + Ur[row * KC + q].idx];
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+      if (q < nl) acc += lc[q].val * (double)xj[(long)a * m + lc[q].idx];
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+      if (q < nu) acc += uc[q].val * (double)xj[(long)bb * m + uc[q].idx];
     tbuf[j * cs + row] = (V)((double)rw[j * cs + row] - acc);
   }
 }
@@ -842,7 +852,8 @@ __global__ __launch_bounds__(256) void k_copy_blocks(long mm, T* const* __restri
 }
 
 // Level 0, kept lines: rw[k] -= L_k y[e1] + U_k y[e2] with the sparse couplings and
-// y[e] = Y_e rw[e] (so P1 rw[e1] = L_k Y_e1 rw[e1] without reading P1).  Thread per row.
+// y[e] = Y_e rw[e] (so P1 rw[e1] = L_k Y_e1 rw[e1] without reading P1).  Thread per
+// row; the row's couplings are loaded once for all columns.
 template <typename V, int NC>
 __global__ __launch_bounds__(256) void k_fwd_sparse(int m, const int* __restrict__ k_line,
                                                     const int* __restrict__ e1, const int* __restrict__ e2,
@@ -854,14 +865,23 @@ __global__ __launch_bounds__(256) void k_fwd_sparse(int m, const int* __restrict
   if (r >= m) return;
   const long row = (long)k_line[b] * m + r;
   const int a = e1[b], c = e2[b];
+  const int nl = a >= 0 ? nLr[row] : 0, nu = c >= 0 ? nUr[row] : 0;
+  Cpl lc[KC], uc[KC];
+#pragma unroll
+  for (int q = 0; q < KC; ++q) {
+    if (q < nl) lc[q] = Lr[row * KC + q];
+    if (q < nu) uc[q] = Ur[row * KC + q];
+  }
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     const V* yj = y + j * cs;
     double acc = 0.0;
-    if (a >= 0)
-      for (int q = 0; q < nLr[row]; ++q) acc += Lr[row * KC + q].val * (double)yj[(long)a * m + Lr[row * KC + q].idx];
-    if (c >= 0)
-      for (int q = 0; q < nUr[row]; ++q) acc += Ur[row * KC + q].val * (double)yj[(long)c * m + Ur[row * KC + q].idx];
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+      if (q < nl) acc += lc[q].val * (double)yj[(long)a * m + lc[q].idx];
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+      if (q < nu) acc += uc[q].val * (double)yj[(long)c * m + uc[q].idx];
     rw[j * cs + row] = (V)((double)rw[j * cs + row] - acc);
   }
 }
