@@ -442,6 +442,35 @@ def kernel_sample(n, reps=10):
     log('kernel sample done: %d applies, %d two-column applies, %d-float copy' % (reps, reps, big))
 
 
+def assembly_sample(reps=3):
+    """Short launch sequence for hardware-counter runs of the assembly at sizes that
+    stream from HBM: ``reps`` rhs + Jacobian evaluations of the 1024 x 1024 cavity and
+    of the 64^3 cavity at Re = 1000 (random state), then the plain copy of
+    kernel_sample as calibration."""
+    import numpy
+    import torch
+    import transiflow_amd as tf
+    from transiflow_amd import _lib
+    from transiflow_amd.vector import ptr, stream
+    for shape in ((1024, 1024, 1), (64, 64, 64)):
+        itf = tf.Interface({'Reynolds Number': 1000.0}, *shape)
+        x = itf.vector_from_array(numpy.random.default_rng(3).uniform(-0.1, 0.1, itf.n))
+        for _ in range(reps):
+            F = itf.rhs(x)
+            J = itf.jacobian(x)
+        torch.cuda.synchronize()
+        log('assembly sample %s: n %d, nnz %d' % ('x'.join(map(str, shape)), itf.n, J.nnz))
+        del itf, x, F, J
+        torch.cuda.empty_cache()
+    big = 64 * 2 ** 20
+    a = torch.ones(big, dtype=torch.float32, device='cuda')
+    b = torch.ones(big, dtype=torch.float32, device='cuda')
+    c = torch.empty(big, dtype=torch.float32, device='cuda')
+    _lib.call('tf_axpby_f32', big, 1.0, ptr(a), -1.0, ptr(b), ptr(c), stream())
+    torch.cuda.synchronize()
+    log('assembly sample done')
+
+
 def conv_sample(reps=5, n=64):
     """Short launch sequence for the learned network's MFMA kernels: the 3D
     cavity's network (4 -> 32 -> 32 -> 4 channels, 3x3x3) on an n^3 grid,
@@ -530,6 +559,8 @@ def main():
     ap.add_argument('--workers', type=int, default=8, help='batch workload: concurrent worker streams')
     ap.add_argument('--kernel-sample', action='store_true',
                     help='run only the short launch sequence used for hardware-counter profiles')
+    ap.add_argument('--assembly-sample', action='store_true',
+                    help='only the assembly launch sequence for hardware-counter runs (1024^2 and 64^3)')
     ap.add_argument('--conv-sample', action='store_true',
                     help="run only the learned network's forward and backward passes at 64^3 (MFMA profiles)")
     args = ap.parse_args()
@@ -543,6 +574,9 @@ def main():
         return
     if args.conv_sample:
         conv_sample()
+        return
+    if args.assembly_sample:
+        assembly_sample()
         return
     if args.workload == 'batch':
         batch_main(args)

@@ -129,8 +129,11 @@ __global__ __launch_bounds__(64 * DOF) void k_assemble(Grid g, Physics ph, const
     if (t == NT - 1) block_sums[blockIdx.x] = s[NT - 1];
   } else {
     // The workgroup's rows are contiguous in the CSR arrays: when its entries fit,
-    // the rows are emitted into LDS and the segment is written out coalesced.
-    constexpr int CAP = NT * 12;
+    // the rows are emitted into LDS and the segment is written out coalesced.  The
+    // capacity covers the mean row length with margin (2D about 8.7 entries per row,
+    // 3D about 14): with 12 per row the 3D segments overflowed and the rows were
+    // written in place, uncoalesced (1.7x the bytes written at 64^3).
+    constexpr int CAP = NT * (DIM == 3 ? 16 : 12);
     __shared__ int s_col[CAP];
     __shared__ double s_val[CAP];
     s[lr] = active ? counts[row] : 0;
