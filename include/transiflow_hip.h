@@ -228,10 +228,10 @@ int tf_bcr_solve2_f32(tf_bcr* h, const float* r0, const float* r1, float* x0, fl
  * count) into a hipGraph and replayed with one launch between two staging kernels (no
  * reference counterpart; launch overhead on small grids and batches). */
 int tf_bcr_set_graph(tf_bcr* h, int on);
-/* on != 0: with the fused backward, the levels with fewer than 12288 rows per launch run
- * as one ordered-task launch instead of one launch per level (no reference counterpart;
- * the launch floor of the upper levels).  Off by default and not in use: its first GPU
- * run did not finish in time and the cause is not established.
+/* on != 0 would run the levels with fewer than 12288 rows per launch as one ordered-task
+ * launch (no reference counterpart; the launch floor of the upper levels); it returns an
+ * error: on the GPU that launch did not finish within the time limit and the cause is not
+ * established.  0 (default) keeps one launch per level.
  * tf_bcr_top_errors: waits of that launch that gave up since the last call (then reset);
  * synchronises the stream. */
 int tf_bcr_set_top(tf_bcr* h, int on);
