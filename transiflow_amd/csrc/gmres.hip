@@ -67,6 +67,10 @@ int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, 
 int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
                   hipStream_t st);
+int cgs2_one_f64(int n, int j, const double* V, double* w, double* work, double* h, double* y, double* v,
+                 hipStream_t st);
+int cgs2_one_f32(int n, int j, const float* V, float* w, double* work, double* h, double* y, float* v,
+                 hipStream_t st);
 }  // namespace tf
 
 namespace {
@@ -159,24 +163,10 @@ int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
     hipLaunchKernelGGL(k_spmv_desc<double>, dim3(tf::spmv_blocks<double>(n)), dim3(256), 0, st, n, k->desc,
                        (const double*)o.Z(j), (double*)k->w);
   TF_LAUNCH_CHECK("fgmres spmv");
-  if (k->single) {
-    const float* Vf = (const float*)k->V;
-    float* wf = (float*)k->w;
-    if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->h, 0, st)) return e;
-    if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->h, wf, -1.0, st)) return e;
-    if (int e = tf_multidot_f32(n, j + 1, Vf, n, wf, k->work, k->y, 0, st)) return e;
-    if (int e = tf_multiaxpy_f32(n, j + 1, Vf, n, k->y, wf, -1.0, st)) return e;
-  } else {
-    const double* Vd = (const double*)k->V;
-    double* wd = (double*)k->w;
-    if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->h, 0, st)) return e;
-    if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->h, wd, -1.0, st)) return e;
-    if (int e = tf_multidot_f64(n, j + 1, Vd, n, wd, k->work, k->y, 0, st)) return e;
-    if (int e = tf_multiaxpy_f64(n, j + 1, Vd, n, k->y, wd, -1.0, st)) return e;
-  }
-  if (int e = tf_axpby_f64(j + 1, 1.0, k->h, 1.0, k->y, k->h, st)) return e;
-  if (int e = k->single ? tf_normalize_f32(n, (const float*)k->w, k->work, k->h + j + 1, (float*)o.V(j + 1), st)
-                        : tf_normalize_f64(n, (const double*)k->w, k->work, k->h + j + 1, (double*)o.V(j + 1), st))
+  if (int e = k->single ? tf::cgs2_one_f32(n, j, (const float*)k->V, (float*)k->w, k->work, k->h, k->y,
+                                           (float*)o.V(j + 1), st)
+                        : tf::cgs2_one_f64(n, j, (const double*)k->V, (double*)k->w, k->work, k->h, k->y,
+                                           (double*)o.V(j + 1), st))
     return e;
   TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
   return 0;

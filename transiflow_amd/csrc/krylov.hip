@@ -141,25 +141,14 @@ __global__ __launch_bounds__(MD_THREADS) void k_multidot_final(int k, int nb, co
   }
 }
 
-// w += sign * sum_i coef[i] V[i]   (coef on the device, FP64; accumulation in
-// FP64 in the order i = 0, 1, ...).  VEC: 16-byte loads, four basis vectors'
-// loads issued together.
+// w += sum_i c[i] V[i] with the coefficients c (sign applied) in LDS: accumulation
+// in FP64 in the order i = 0, 1, ...; VEC: 16-byte loads, four basis vectors' loads
+// issued together.  Grid-stride over the entries.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __restrict__ V, long ldv,
-                                                   const double* __restrict__ coef, T* __restrict__ w, double sign,
-                                                   const T* __restrict__ V1 = nullptr,
-                                                   const double* __restrict__ coef1 = nullptr,
-                                                   T* __restrict__ w1 = nullptr) {
-  if (blockIdx.y) {
-    V = V1;
-    coef = coef1;
-    w = w1;
-  }
+__device__ __forceinline__ void axpy_body(int n, int k, const T* __restrict__ V, long ldv, const double* c,
+                                          T* __restrict__ w) {
   using VT = typename Vec16<T>::type;
   constexpr int VN = VEC ? Vec16<T>::N : 1;
-  __shared__ double c[MAX_BASIS];
-  for (int i = threadIdx.x; i < k; i += blockDim.x) c[i] = sign * coef[i];
-  __syncthreads();
   const int units = n / VN;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < units; e += gridDim.x * blockDim.x) {
     if constexpr (VEC) {
@@ -189,6 +178,104 @@ __global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __rest
       double acc = (double)w[e];
       for (int i = 0; i < k; ++i) acc += c[i] * (double)V[(long)i * ldv + e];
       w[e] = (T)acc;
+    }
+  }
+}
+
+// w += sign * sum_i coef[i] V[i]   (coef on the device, FP64).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void k_multiaxpy(int n, int k, const T* __restrict__ V, long ldv,
+                                                   const double* __restrict__ coef, T* __restrict__ w, double sign,
+                                                   const T* __restrict__ V1 = nullptr,
+                                                   const double* __restrict__ coef1 = nullptr,
+                                                   T* __restrict__ w1 = nullptr) {
+  if (blockIdx.y) {
+    V = V1;
+    coef = coef1;
+    w = w1;
+  }
+  __shared__ double c[MAX_BASIS];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) c[i] = sign * coef[i];
+  __syncthreads();
+  axpy_body<T, VEC>(n, k, V, ldv, c, w);
+}
+
+// ---- CGS2 with the block sums reduced where they are consumed --------------------
+//
+// The Arnoldi step's reductions for small vectors: instead of a separate kernel
+// that sums the multidot's per-block partials, every block of the consuming kernel
+// stages the k x nb partials in LDS and sums them itself (the same values in every
+// block: for each i, b ascending, in FP64); block 0 stores them.  Used when the
+// partials fit FIN_LDS doubles (n up to about 0.4M at restart 30), where the
+// launches, not the bytes, set the step's time.
+constexpr int FIN_LDS = 6144;
+
+__device__ __forceinline__ void fin_sums(int k, int nb, const double* __restrict__ partial, double* sp, double* c) {
+  for (int e = threadIdx.x; e < k * nb; e += blockDim.x) sp[e] = partial[e];
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += sp[i * nb + b];
+    c[i] = s;
+  }
+  __syncthreads();
+}
+
+// c = the sums of partial (k x nb); block 0: hout[i] = c[i] (add: hout[i] += c[i]);
+// then w -= sum_i c[i] V[i].  blockIdx.y 1 takes the second system.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void k_multiaxpy_fin(int n, int k, int nb, const T* __restrict__ V, long ldv,
+                                                       const double* __restrict__ partial, double* __restrict__ hout,
+                                                       int add, T* __restrict__ w, const T* __restrict__ V1,
+                                                       const double* __restrict__ partial1,
+                                                       double* __restrict__ hout1, T* __restrict__ w1) {
+  if (blockIdx.y) {
+    V = V1;
+    partial = partial1;
+    hout = hout1;
+    w = w1;
+  }
+  __shared__ double sp[FIN_LDS];
+  __shared__ double c[MAX_BASIS];
+  fin_sums(k, nb, partial, sp, c);
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < k; i += blockDim.x) hout[i] = add ? hout[i] + c[i] : c[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) c[i] = -c[i];
+  __syncthreads();
+  axpy_body<T, VEC>(n, k, V, ldv, c, w);
+}
+
+// nrm = sqrt(sum of partial[0..nb)) (block 0 stores it), out = w / nrm
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void k_scale_fin(int n, int nb, const double* __restrict__ partial,
+                                                   double* __restrict__ nrm_out, const T* __restrict__ w,
+                                                   T* __restrict__ out, const double* __restrict__ partial1,
+                                                   double* __restrict__ nrm_out1, const T* __restrict__ w1,
+                                                   T* __restrict__ out1) {
+  if (blockIdx.y) {
+    partial = partial1;
+    nrm_out = nrm_out1;
+    w = w1;
+    out = out1;
+  }
+  __shared__ double sp[FIN_LDS];
+  __shared__ double c[1];
+  fin_sums(1, nb, partial, sp, c);
+  const double nrm = sqrt(c[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm_out = nrm;
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = VEC ? Vec16<T>::N : 1;
+  const double inv = 1.0 / nrm;
+  const int units = n / VN;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < units; e += gridDim.x * blockDim.x) {
+    if constexpr (VEC) {
+      VT v = reinterpret_cast<const VT*>(w)[e];
+#pragma unroll
+      for (int l = 0; l < VN; ++l) v[l] = (T)((double)v[l] * inv);
+      reinterpret_cast<VT*>(out)[e] = v;
+    } else {
+      out[e] = (T)((double)w[e] * inv);
     }
   }
 }
@@ -414,15 +501,32 @@ __global__ void k_add_pair(int k, double* h0, const double* y0, double* h1, cons
   for (int i = threadIdx.x; i < k; i += blockDim.x) h[i] = h[i] + y[i];
 }
 
+// One CGS2 orthogonalisation and normalisation of step j for nsys (1 or 2) systems
+// in the same launches (blockIdx.y picks the system; the second system's pointers
+// are unused for one): h = V^T w, w -= V h, y = V^T w, w -= V y, h += y,
+// h[j+1] = |w|, V[j+1] = w / |w|.  With the partials in LDS reach (k nb <= FIN_LDS)
+// the sums are reduced by the consuming kernels (k_multiaxpy_fin, k_scale_fin:
+// five launches); else by separate kernels (k_multidot_final, k_add_pair,
+// k_norm_final: eight).  The choice depends on n and j only, so one system and
+// two systems take the same arithmetic.
 template <typename T>
-int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work0, double* work1, double* h0,
-              double* h1, double* y0, double* y1, T* v0, T* v1, hipStream_t st) {
+int cgs2_sys(int nsys, int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work0, double* work1,
+             double* h0, double* h1, double* y0, double* y1, T* v0, T* v1, hipStream_t st) {
   const int k = j + 1;
-  if (k < 1 || k + 1 > MAX_BASIS) return tf::fail("cgs2 pair: bad basis size");
+  if (k < 1 || k + 1 > MAX_BASIS) return tf::fail("cgs2: bad basis size");
+  if (nsys == 1) {
+    V1 = V0;
+    w1 = w0;
+    work1 = work0;
+    h1 = h0;
+    y1 = y0;
+    v1 = v0;
+  }
   const int nb = md_blocks(n);
   const bool vec = vec_ok<T>(n, n, V0, w0) && vec_ok<T>(n, n, V1, w1) && vec_ok<T>(n, n, v0, v1);
-  const dim3 md(nb, 2), blk(MD_THREADS), fin(k, 2);
-  const dim3 ew(grid_for(vec ? n / Vec16<T>::N : n), 2);
+  const bool fused = (long)k * nb <= FIN_LDS;
+  const dim3 md(nb, nsys), blk(MD_THREADS), fin(k, nsys);
+  const dim3 ew(grid_for(vec ? n / Vec16<T>::N : n), nsys);
   for (int pass = 0; pass < 2; ++pass) {
     double* c0 = pass ? y0 : h0;
     double* c1 = pass ? y1 : h1;
@@ -432,6 +536,16 @@ int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work
     else
       hipLaunchKernelGGL((k_multidot_partial<T, false>), md, blk, 0, st, n, k, V0, (long)n, (const T*)w0, work0, V1,
                          (const T*)w1, work1);
+    if (fused) {
+      // pass 0: h = sums; pass 1: h += sums (y is not needed further)
+      if (vec)
+        hipLaunchKernelGGL((k_multiaxpy_fin<T, true>), ew, dim3(256), 0, st, n, k, nb, V0, (long)n,
+                           (const double*)work0, h0, pass, w0, V1, (const double*)work1, h1, w1);
+      else
+        hipLaunchKernelGGL((k_multiaxpy_fin<T, false>), ew, dim3(256), 0, st, n, k, nb, V0, (long)n,
+                           (const double*)work0, h0, pass, w0, V1, (const double*)work1, h1, w1);
+      continue;
+    }
     hipLaunchKernelGGL(k_multidot_final, fin, blk, 0, st, k, nb, (const double*)work0, c0, 0, (const double*)work1,
                        c1);
     if (vec)
@@ -441,7 +555,7 @@ int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work
       hipLaunchKernelGGL((k_multiaxpy<T, false>), ew, dim3(256), 0, st, n, k, V0, (long)n, (const double*)c0, w0,
                          -1.0, V1, (const double*)c1, w1);
   }
-  hipLaunchKernelGGL(k_add_pair, dim3(2), dim3(64), 0, st, k, h0, (const double*)y0, h1, (const double*)y1);
+  if (!fused) hipLaunchKernelGGL(k_add_pair, dim3(nsys), dim3(64), 0, st, k, h0, (const double*)y0, h1, (const double*)y1);
   // |w| into h[j+1], V[j+1] = w / |w|
   if (vec)
     hipLaunchKernelGGL((k_multidot_partial<T, true>), md, blk, 0, st, n, 1, (const T*)w0, (long)n, (const T*)w0,
@@ -449,7 +563,17 @@ int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work
   else
     hipLaunchKernelGGL((k_multidot_partial<T, false>), md, blk, 0, st, n, 1, (const T*)w0, (long)n, (const T*)w0,
                        work0, (const T*)w1, (const T*)w1, work1);
-  hipLaunchKernelGGL(k_norm_final, dim3(1, 2), blk, 0, st, nb, (const double*)work0, h0 + k, (const double*)work1,
+  if (fused) {
+    if (vec)
+      hipLaunchKernelGGL((k_scale_fin<T, true>), ew, dim3(256), 0, st, n, nb, (const double*)work0, h0 + k,
+                         (const T*)w0, v0, (const double*)work1, h1 + k, (const T*)w1, v1);
+    else
+      hipLaunchKernelGGL((k_scale_fin<T, false>), ew, dim3(256), 0, st, n, nb, (const double*)work0, h0 + k,
+                         (const T*)w0, v0, (const double*)work1, h1 + k, (const T*)w1, v1);
+    TF_LAUNCH_CHECK("cgs2");
+    return 0;
+  }
+  hipLaunchKernelGGL(k_norm_final, dim3(1, nsys), blk, 0, st, nb, (const double*)work0, h0 + k, (const double*)work1,
                      h1 + k);
   if (vec)
     hipLaunchKernelGGL((k_scale_by_inv<T, true>), ew, dim3(256), 0, st, n, (const T*)w0, (const double*)(h0 + k), v0,
@@ -457,26 +581,33 @@ int cgs2_pair(int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work
   else
     hipLaunchKernelGGL((k_scale_by_inv<T, false>), ew, dim3(256), 0, st, n, (const T*)w0, (const double*)(h0 + k),
                        v0, (const T*)w1, (const double*)(h1 + k), v1);
-  TF_LAUNCH_CHECK("cgs2 pair");
+  TF_LAUNCH_CHECK("cgs2");
   return 0;
 }
 
 }  // namespace
 
 namespace tf {
-// One CGS2 Arnoldi orthogonalisation and normalisation for two systems in the
-// same launches (each kernel's blockIdx.y picks the system): for each,
-// h = V^T w, w -= V h, y = V^T w, w -= V y, h += y, h[j+1] = |w|, V[j+1] = w / |w|,
-// with the arithmetic of the single-system building blocks.
+// One CGS2 Arnoldi orthogonalisation and normalisation (cgs2_sys) for two systems
+// in the same launches, or for one; the same arithmetic either way.
 int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, double* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, double* v0, double* v1,
                   hipStream_t st) {
-  return cgs2_pair<double>(n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+  return cgs2_sys<double>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
 }
 int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
                   hipStream_t st) {
-  return cgs2_pair<float>(n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+  return cgs2_sys<float>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+}
+// the same for one system
+int cgs2_one_f64(int n, int j, const double* V, double* w, double* work, double* h, double* y, double* v,
+                 hipStream_t st) {
+  return cgs2_sys<double>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, st);
+}
+int cgs2_one_f32(int n, int j, const float* V, float* w, double* work, double* h, double* y, float* v,
+                 hipStream_t st) {
+  return cgs2_sys<float>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, st);
 }
 }  // namespace tf
 
