@@ -242,11 +242,10 @@ class LineFactor:
             raise SolverError('line factorisation: matrix is not block tridiagonal over grid lines (flags %d)'
                               % flags[0])
         self.perturbed = flags[1]
-        self.check_applies(st)
 
     def check_applies(self, st):
-        """Raise if a wait of the upper-level launch of an apply since the last
-        check gave up (synchronises ``st``)."""
+        """Raise if a wait of the one-launch upper levels (tf_bcr_set_top, which is
+        disabled) gave up since the last check (synchronises ``st``)."""
         if self._p == 'tf_bcr':
             n = ctypes.c_int()
             _lib.call('tf_bcr_top_errors', self._h, ctypes.byref(n), st)
