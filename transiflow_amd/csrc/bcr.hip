@@ -1571,7 +1571,7 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   for (Cpl** c : {&h->Lr, &h->Ur, &h->Ut, &h->Lt}) TF_HIP(hipMalloc(c, sizeof(Cpl) * KC * (size_t)n));
   for (int** c : {&h->nLr, &h->nUr, &h->nUt, &h->nLt}) TF_HIP(hipMalloc(c, sizeof(int) * (size_t)n));
   TF_HIP(hipMalloc(&h->flags, sizeof(int) * 2));
-  TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 2, hipHostMallocDefault));
+  TF_HIP(hipHostMalloc(&h->host_flags, sizeof(int) * 3, hipHostMallocDefault));  // [err, nperturb, top waits]
   TF_HIP(hipMalloc(&h->rw, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->tb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
@@ -1889,15 +1889,16 @@ int tf_bcr_set_small(tf_bcr* h, int on) {
 int tf_bcr_top_errors(tf_bcr* h, int* count_host, void* stream) {
   if (!h || !count_host) return tf::fail("null argument");
   hipStream_t st = (hipStream_t)stream;
-  unsigned v = 0;
-  TF_HIP(hipMemcpyAsync(&v, h->topsync + 2 + MAX_TOP, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  // through the handle's pinned host words (a copy to pageable memory can stall other
+  // threads' stream captures) and a memset on the stream
+  TF_HIP(hipMemcpyAsync(h->host_flags + 2, h->topsync + 2 + MAX_TOP, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   TF_HIP(hipStreamSynchronize(st));
+  const int v = h->host_flags[2];
   if (v) {
-    const unsigned zero = 0;
-    TF_HIP(hipMemcpyAsync(h->topsync + 2 + MAX_TOP, &zero, sizeof(unsigned), hipMemcpyHostToDevice, st));
+    TF_HIP(hipMemsetAsync(h->topsync + 2 + MAX_TOP, 0, sizeof(unsigned), st));
     TF_HIP(hipStreamSynchronize(st));
   }
-  *count_host = (int)v;
+  *count_host = v;
   return 0;
 }
 
