@@ -502,6 +502,11 @@ def conv_sample(reps=5, n=64):
     print(json.dumps({'conv_sample': out, 'cells': ncell}), flush=True)
 
 
+def _lib_fallbacks():
+    from transiflow_amd import _lib
+    return _lib.load().tf_capture_fallbacks()
+
+
 def batch_main(args):
     """64 cavity instances (square grids drawn from 32..256 step 32, target Re
     uniform in [100, 1500], numpy default_rng(2)), each Newton at Re = 0 then
@@ -528,6 +533,7 @@ def batch_main(args):
                                '(default_rng(2)), continuation from Re 0, ds 100, tol 1e-8, %d streams' % args.workers},
         'one_stream_steps_per_s': acc / seq_wall, 'speedup_vs_one_stream': seq_wall / wall,
         'max_final_state_rel_diff_vs_one_stream': err,
+        'graph_captures_run_eagerly': int(_lib_fallbacks()),
         'instances': [{'n': i['n'], 'Re': i['Re'], 'accepted': r['accepted'], 'seconds': r['seconds']}
                       for i, r in zip(inst, bat)]}), flush=True)
     del torch

@@ -312,6 +312,10 @@ int tf_gemm_table(int* out_host, int max_rows);
 
 /* ---- opt-in kernel timing with HIP events (no reference counterpart; used by bench.py) ---- */
 int tf_prof_enable(int on);
+/* Graph captures (line-factor applies, Arnoldi steps) that failed -- work of another
+ * thread can invalidate a capture -- and whose call ran eagerly with the same kernels
+ * instead, since the process started (no reference counterpart). */
+int tf_capture_fallbacks(void);
 int tf_prof_read(int id, long* count, double* ms_total);
 int tf_prof_reset(void);
 int tf_prof_work(int id, double* work_total);

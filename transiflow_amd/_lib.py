@@ -22,6 +22,7 @@ SIGNATURES = {
     'tf_device_count': (_c_int, []),
     'tf_prof_enable': (_c_int, [_c_int]),
     'tf_prof_read': (_c_int, [_c_int, ctypes.POINTER(_c_long), _dp]),
+    'tf_capture_fallbacks': (_c_int, []),
     'tf_prof_reset': (_c_int, []),
     'tf_prof_work': (_c_int, [_c_int, ctypes.POINTER(ctypes.c_double)]),
     'tf_problem_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int,

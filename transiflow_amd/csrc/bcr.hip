@@ -1452,21 +1452,24 @@ int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream
     // non-blocking: a capture must not order against the legacy stream that other
     // threads use for synchronous copies
     if (!h->cap) TF_HIP(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
-    TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
-    run_levels<T, V, NC>(h, rw, xw, h->cap);
-    const hipError_t le = hipGetLastError();
     hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(h->cap, &g);
-    if (le != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return tf::fail_hip(le, "bcr apply capture");
-    }
-    TF_HIP(ce);
     hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(g);
-      return tf::fail_hip(ie, "hipGraphInstantiate (line-factor apply)");
+    hipError_t le = hipSuccess, ce = hipSuccess, ie = hipSuccess;
+    {
+      std::lock_guard<std::mutex> lock(tf::capture_mutex());
+      TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+      run_levels<T, V, NC>(h, rw, xw, h->cap);
+      le = hipGetLastError();
+      ce = hipStreamEndCapture(h->cap, &g);
+      if (le == hipSuccess && ce == hipSuccess && g) ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    }
+    if (le != hipSuccess || ce != hipSuccess || !g || ie != hipSuccess) {
+      // a failed capture or instantiation: this apply runs eagerly, with the same
+      // kernels, and is captured again next time
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      tf::note_capture_fallback();
+      return solve_impl<T, V, NC>(h, r, x, st);
     }
     gs.graph = g;
     gs.exec = ex;
@@ -1577,7 +1580,12 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->xw, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->topsync, sizeof(unsigned) * TOP_SYNC_WORDS));
-  TF_HIP(hipMemset(h->topsync, 0, sizeof(unsigned) * TOP_SYNC_WORDS));
+  {
+    // zeroed through the library's non-blocking upload stream: a hipMemset on the legacy
+    // stream fails, or invalidates the capture, while another thread captures a graph
+    const unsigned zeros[TOP_SYNC_WORDS] = {};
+    TF_HIP(tf::upload(h->topsync, zeros, sizeof(zeros)));
+  }
   h->use_top = 0;  // off: see tf_bcr_set_top
   h->use_small = 1;
 

@@ -6,6 +6,7 @@
 #include "spmv.h"
 #include "tf_common.h"
 
+#include <atomic>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -67,6 +68,14 @@ void prof_work(int id, double w) {
   ProfState& s = ps();
   std::lock_guard<std::mutex> g(s.mu);
   s.work[id] += w;
+}
+
+static std::atomic<int> g_capture_fallbacks{0};
+void note_capture_fallback() { ++g_capture_fallbacks; }
+int capture_fallbacks() { return g_capture_fallbacks.load(); }
+std::mutex& capture_mutex() {
+  static std::mutex m;
+  return m;
 }
 
 void prof_begin(int id, hipStream_t st) {
@@ -324,6 +333,9 @@ int tf_prof_enable(int on) {
 }
 
 // Synchronises the recorded events; count/ms_total receive the totals of kernel id.
+// Graph captures that failed and whose call ran eagerly instead, since the process started.
+int tf_capture_fallbacks(void) { return tf::capture_fallbacks(); }
+
 int tf_prof_read(int id, long* count, double* ms_total) {
   if (id < 0 || id >= tf::PROF_IDS) return tf::fail("bad profiling id");
   tf::ProfState& s = tf::ps();

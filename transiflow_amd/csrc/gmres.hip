@@ -176,20 +176,24 @@ int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
 int arnoldi_step_graph(tf_krylov* k, int j, hipStream_t st) {
   if (!k->step[j]) {
     if (!k->cap) TF_HIP(hipStreamCreateWithFlags(&k->cap, hipStreamNonBlocking));
-    TF_HIP(hipStreamBeginCapture(k->cap, hipStreamCaptureModeThreadLocal));
-    const int e = arnoldi_step(k, j, k->cap);
     hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(k->cap, &g);
-    if (e) {
-      if (g) (void)hipGraphDestroy(g);
-      return e;
-    }
-    TF_HIP(ce);
     hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(g);
-      return tf::fail_hip(ie, "hipGraphInstantiate (Arnoldi step)");
+    int e = 0;
+    hipError_t ce = hipSuccess, ie = hipSuccess;
+    {
+      std::lock_guard<std::mutex> lock(tf::capture_mutex());
+      TF_HIP(hipStreamBeginCapture(k->cap, hipStreamCaptureModeThreadLocal));
+      e = arnoldi_step(k, j, k->cap);
+      ce = hipStreamEndCapture(k->cap, &g);
+      if (!e && ce == hipSuccess && g) ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    }
+    if (e || ce != hipSuccess || !g || ie != hipSuccess) {
+      // a failed capture or instantiation: this step runs eagerly, with the same
+      // kernels, and is captured again next time
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      tf::note_capture_fallback();
+      return arnoldi_step(k, j, st);
     }
     k->step_graph[j] = g;
     k->step[j] = ex;
@@ -238,20 +242,23 @@ int arnoldi_step2_graph(tf_krylov* a, tf_krylov* b, int j, hipStream_t st) {
   }
   if (!a->step2[j]) {
     if (!a->cap) TF_HIP(hipStreamCreateWithFlags(&a->cap, hipStreamNonBlocking));
-    TF_HIP(hipStreamBeginCapture(a->cap, hipStreamCaptureModeThreadLocal));
-    const int e = arnoldi_step2(a, b, j, a->cap);
     hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(a->cap, &g);
-    if (e) {
-      if (g) (void)hipGraphDestroy(g);
-      return e;
-    }
-    TF_HIP(ce);
     hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(g);
-      return tf::fail_hip(ie, "hipGraphInstantiate (two-system Arnoldi step)");
+    int e = 0;
+    hipError_t ce = hipSuccess, ie = hipSuccess;
+    {
+      std::lock_guard<std::mutex> lock(tf::capture_mutex());
+      TF_HIP(hipStreamBeginCapture(a->cap, hipStreamCaptureModeThreadLocal));
+      e = arnoldi_step2(a, b, j, a->cap);
+      ce = hipStreamEndCapture(a->cap, &g);
+      if (!e && ce == hipSuccess && g) ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    }
+    if (e || ce != hipSuccess || !g || ie != hipSuccess) {
+      // as in arnoldi_step_graph: this step runs eagerly and is captured again next time
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      tf::note_capture_fallback();
+      return arnoldi_step2(a, b, j, st);
     }
     a->step2_graph[j] = g;
     a->step2[j] = ex;

@@ -1,6 +1,8 @@
 // Shared host-side helpers for the TransiFlow HIP backend's C ABI.
 #pragma once
 
+#include <mutex>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -53,6 +55,13 @@ hipError_t upload(void* dst, const void* src, size_t bytes);
 void prof_begin(int id, hipStream_t st);
 void prof_end(int id, hipStream_t st);
 void prof_work(int id, double w);
+// A graph capture that failed (another thread's work can invalidate it) and whose
+// call ran eagerly instead: counted for tf_capture_fallbacks().
+void note_capture_fallback();
+int capture_fallbacks();
+// Held around every stream capture and graph instantiation of the library, so that
+// captures of different threads (one per worker stream) never overlap.
+std::mutex& capture_mutex();
 
 }  // namespace tf
 
