@@ -255,6 +255,13 @@ int tf_bcr_set_gemm_split(tf_bcr* h, int split);
  * is one kernel (x_e = Y r_e - (Y L) x_a - (Y U) x_b) instead of two (no reference counterpart:
  * worth it for factors applied many times, such as the lagged FP32 factor of the mixed solver). */
 int tf_bcr_set_fused(tf_bcr* h, int on);
+/* lines in 2..4 (fused factors): once the reduction is down to at most that many lines, each
+ * factorisation also forms the dense inverse Z of their reduced system (about 20 m x m x m
+ * products at four lines; FP16 copies for single != 0), and each apply maps those lines'
+ * right-hand sides to their solutions in one launch instead of the forward and backward levels
+ * above them (five launches at four lines).  0 (default): off.  Allocates Z; takes effect at the
+ * next factorisation (no reference counterpart: launch latency of the few-line levels). */
+int tf_bcr_set_dense_top(tf_bcr* h, int lines);
 
 /* ---- block ILU over grid x-lines, four-colour line order (two colours in 2D) ----
  * M = (P + L) P^-1 (P + U) with dense m x m pivots P (m = nx dof) and the Jacobian's own

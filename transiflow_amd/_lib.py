@@ -109,6 +109,7 @@ SIGNATURES = {
     'tf_bcr_set_half': (_c_int, [_vp, _c_int, _vp]),
     'tf_bcr_set_gemm_split': (_c_int, [_vp, _c_int]),
     'tf_bcr_set_fused': (_c_int, [_vp, _c_int]),
+    'tf_bcr_set_dense_top': (_c_int, [_vp, _c_int]),
     'tf_lineilu_create': (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_int, _c_int]),
     'tf_lineilu_destroy': (_c_int, [_vp]),
     'tf_lineilu_factor': (_c_int, [_vp, _vp, _vp, _vp, _c_int, _vp]),

@@ -37,6 +37,7 @@
 #include "tf_common.h"
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <type_traits>
 #include <vector>
@@ -711,6 +712,48 @@ __global__ __launch_bounds__(256) void k_small(int m, int G, int vec, int dir, S
   }
 }
 
+// ---- the top of the reduction as one dense map ------------------------------------
+//
+// Once the reduction is down to at most TOPD_MAX lines (the lines active at level
+// zl), their solutions are a linear map of their right-hand sides after the forward
+// levels below zl: x[line i] = sum_j Z[i][j] rw[line j], with Z the inverse of that
+// reduced system, formed at factor time from the factor's own blocks
+// (tf_bcr_set_dense_top).  One launch replaces the forward levels from zl up and the
+// backward levels down to zl (five launches at four lines), for four more blocks
+// read at four lines (16 instead of 12).
+constexpr int TOPD_MAX = 4;
+
+template <typename B>
+struct DenseTop {
+  int n;
+  int line[TOPD_MAX];
+  typename RowOf<B>::type blk[TOPD_MAX][TOPD_MAX];  // [output line][input line]
+};
+
+template <typename B, typename V, int NC>
+__global__ __launch_bounds__(256) void k_dense_top(int m, int G, int vec, DenseTop<B> a, const V* __restrict__ rw,
+                                                   V* __restrict__ xo, long cs) {
+  const int b = blockIdx.y;
+  const int g = threadIdx.x % G;
+  const int r = blockIdx.x * rows_per_block(G) + (int)threadIdx.x / G;
+  const bool live = r < m;
+  const int rr = live ? r : 0;
+  double acc[NC], d[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
+#pragma unroll
+  for (int t = 0; t < TOPD_MAX; ++t) {
+    if (t < a.n) {
+      small_dot<NC>(a.blk[b][t], rr, rw + (long)a.line[t] * m, cs, m, g, G, vec, d);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) acc[j] += d[j];
+    }
+  }
+  if (live && g == 0)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) xo[j * cs + (long)a.line[b] * m + r] = (V)acc[j];
+}
+
 // ---- the upper levels in one launch ---------------------------------------------
 //
 // The levels with few rows (batch x m < TOP_ROWS: the last forward levels and the
@@ -1012,6 +1055,14 @@ struct BcrGraph {
   hipGraphExec_t exec = nullptr;
 };
 
+// One step of forming the dense top Z at factor time: kind 0 zeroes the blocks of
+// list a, 1 copies a[i] -> b[i], 2 is the batched product c = alpha a b + beta c.
+struct BcrZStep {
+  int kind, count;
+  void **a, **b, **c;
+  double alpha, beta;
+};
+
 struct tf_bcr {
   int n, m, N, single;
   Order order;
@@ -1058,6 +1109,16 @@ struct tf_bcr {
   std::vector<char> pool_host, hpool_host;  // host copies of pool and hpool (k_small's operands)
   unsigned* topsync;   // k_top's counters and error word (TOP_SYNC_WORDS)
   void* W;             // scratch of the fused products (largest level, m^2 each)
+  // dense top (tf_bcr_set_dense_top): Z, the inverse of the reduced system of the
+  // zn <= TOPD_MAX lines active at level zl, block (i, j) at (i zn + j) m^2
+  int zmax;                     // requested line limit (0: off)
+  int zl, zn;                   // zn == 0: no dense top
+  std::vector<int> zline;       // the lines of level zl in position order
+  void* Z;                      // factor precision (also read by the FP16 apply)
+  void* zpool;                  // device lists of the steps
+  std::vector<BcrZStep> zsteps;
+  long zreplaced;               // blocks of the levels the dense top replaces in an apply
+  int z_ok;                     // Z holds the inverse for the current factor
 };
 
 namespace {
@@ -1119,6 +1180,15 @@ void invert_batch(tf_bcr* h, void** Xp, int batch, hipStream_t st) {
                        (const T*)Tb);
   }
   (void)Pinv; (void)Kr; (void)Kc; (void)Tb;
+}
+
+// the captured applies (every vector type and column count) are made again on next use
+void drop_graphs(tf_bcr* h) {
+  for (BcrGraph* g : {&h->g64[0], &h->g64[1], &h->g32[0], &h->g32[1]}) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    *g = BcrGraph{};
+  }
 }
 
 template <typename T>
@@ -1200,6 +1270,22 @@ int factor_impl(tf_bcr* h, const int* rp, const int* ci, const double* v, int ga
       hipLaunchKernelGGL(k_to_half, dim3((m + 3) / 4, h->ncv), dim3(256), 0, st, m, (float* const*)h->cv_src,
                          (f16* const*)h->cv_dst, (float* const*)h->cv_scl);
   }
+  // the dense top reads Y L and Y U of the levels from zl up: fused factors only
+  const int zok = (h->zn > 0 && h->fused) ? 1 : 0;
+  if (zok) {
+    const long mm = (long)m * m;
+    for (const BcrZStep& s : h->zsteps) {
+      if (s.kind == 0)
+        hipLaunchKernelGGL(k_zero_many<T>, dim3(256, s.count), dim3(256), 0, st, mm, (T* const*)s.a);
+      else if (s.kind == 1)
+        hipLaunchKernelGGL(k_copy_blocks<T>, dim3(256, s.count), dim3(256), 0, st, mm, (T* const*)s.a,
+                           (T* const*)s.b);
+      else
+        gemm<T>(m, m, m, s.a, m, s.b, m, s.c, m, s.alpha, s.beta, s.count, st);
+    }
+  }
+  if (zok != h->z_ok) drop_graphs(h);  // the captured sweeps take the other path
+  h->z_ok = zok;
   TF_LAUNCH_CHECK("bcr factor");
   return 0;
 }
@@ -1245,6 +1331,40 @@ bool small_level(const tf_bcr* h, int batch, const int* line, const int* n1, con
   return true;
 }
 
+// the dense top's block (i, j) as rows of the factor's storage
+template <typename T>
+bool dense_row(const tf_bcr* h, const Blocks<T>*, int i, int j, typename RowOf<Blocks<T>>::type& r) {
+  if (sizeof(T) != h->es || !h->Z) return false;
+  r.p = (const T*)h->Z + (size_t)h->m * h->m * (size_t)(i * h->zn + j);
+  return true;
+}
+
+// the rows the dense top is read through: the factor's storage, also under the FP16
+// apply (FP16 copies of Z made a weaker preconditioner: about 13% more inner
+// iterations in the 256 x 256 cavity continuation)
+template <typename B>
+struct ZRows {
+  typedef B type;
+};
+template <>
+struct ZRows<HalfBlocks> {
+  typedef Blocks<float> type;
+};
+
+// k_dense_top's operands; false when the handle holds no dense top for this factor
+template <typename B>
+bool dense_top(const tf_bcr* h, DenseTop<B>& a) {
+  a = DenseTop<B>{};
+  if (!h->z_ok || h->zn < 2 || h->zn > TOPD_MAX || (int)h->zline.size() != h->zn) return false;
+  a.n = h->zn;
+  for (int i = 0; i < h->zn; ++i) {
+    a.line[i] = h->zline[i];
+    for (int j = 0; j < h->zn; ++j)
+      if (!dense_row(h, (const B*)nullptr, i, j, a.blk[i][j])) return false;
+  }
+  return true;
+}
+
 // forward and backward sweeps over the levels, rows read through the block
 // source B, for NC columns at stride cs in rw, tb, xo (and the handle's yb)
 template <typename B, typename V, int NC>
@@ -1266,8 +1386,11 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
     return c;
   };
   while (lt < nlev && steps_from(lt) > MAX_TOP) ++lt;
+  typedef typename ZRows<B>::type ZB;
+  DenseTop<ZB> zt;
+  const bool dense = dense_top<ZB>(h, zt);
   TopPlan<B> tp{};
-  if (h->fused && h->use_top && lt < nlev) {
+  if (!dense && h->fused && h->use_top && lt < nlev) {
     auto add = [&](int dir, int batch, const int* line, const int* n1, const int* n2, B b0, B b1, B b2) {
       TopStep<B>& S = tp.s[tp.nsteps++];
       S.dir = dir;
@@ -1294,7 +1417,7 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
     if (tp.nsteps < 2) tp = TopPlan<B>{};
   }
   const bool top = tp.nsteps >= 2;
-  const int fwd_end = top ? lt : nlev;
+  const int fwd_end = dense ? h->zl : (top ? lt : nlev);
   for (int l = 0; l < fwd_end; ++l) {
     const BcrLevelDev& L = h->lev[l];
     if (!L.nk) continue;
@@ -1324,7 +1447,12 @@ void sweep(tf_bcr* h, int vec, B P1(const BcrLevelDev&), B P2(const BcrLevelDev&
   }
   if (top)
     hipLaunchKernelGGL((k_top<B, V, NC>), dim3(TOP_BLOCKS), blk, 0, st, m, vec, tp, rw, xo, cs, h->topsync);
-  for (int l = top ? lt - 1 : nlev - 1; l >= 0; --l) {
+  if (dense) {
+    const int G = group_for(zt.n);
+    hipLaunchKernelGGL((k_dense_top<ZB, V, NC>), dim3(gx(G, 1), zt.n), blk, 0, st, m, G, vec, zt, (const V*)rw, xo,
+                       cs);
+  }
+  for (int l = dense ? h->zl - 1 : (top ? lt - 1 : nlev - 1); l >= 0; --l) {
     const BcrLevelDev& L = h->lev[l];
     const int G = group_for(L.ne);
     if (SmallLevel<B> sa; l > 0 && h->fused && h->use_small && L.ne <= SMALL_MAX && rows_for<B>(L.ne, m) == 1) {
@@ -1383,10 +1511,17 @@ void levels_half(tf_bcr* h, float* rw, float* tb, float* xo, long cs, hipStream_
       [](const BcrLevelDev& L) { return B{(f16* const*)L.Yh, (float* const*)L.Ys}; }, rw, tb, xo, cs, st);
 }
 
-// algorithmic bytes of an NC-column apply: the factor blocks once, six vectors per column
+// algorithmic bytes of an NC-column apply: the factor blocks once (the dense top's in
+// place of the levels it replaces), six vectors per column
 template <typename T, typename V, int NC>
 double solve_work(const tf_bcr* h) {
-  const double one = (double)((sizeof(T) == 4 && sizeof(V) == 4 && h->half) ? h->solve_bytes_half : h->solve_bytes);
+  const bool hb = sizeof(T) == 4 && sizeof(V) == 4 && h->half;
+  double one = (double)(hb ? h->solve_bytes_half : h->solve_bytes);
+  if (h->z_ok) {
+    // the dense top's blocks (factor precision) in place of those of the levels it replaces
+    const double mm = (double)h->m * h->m;
+    one += (double)h->zn * h->zn * h->es * mm - (double)h->zreplaced * (hb ? 2.0 * mm + 4.0 * h->m : h->es * mm);
+  }
   return one + (NC - 1) * 6.0 * sizeof(V) * (double)h->n;
 }
 
@@ -1830,7 +1965,7 @@ int tf_bcr_destroy(tf_bcr* h) {
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
                   (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W,
-                  (void*)h->topsync})
+                  (void*)h->topsync, h->Z, h->zpool})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;
@@ -2071,6 +2206,131 @@ int tf_bcr_set_fused(tf_bcr* h, int on) {
   if (h->factored && (on != 0) != (h->fused != 0))
     return tf::fail("tf_bcr_set_fused: the handle already holds a factor of the other form");
   h->fused = on ? 1 : 0;
+  return 0;
+}
+
+// lines in 2..TOPD_MAX: once the reduction is down to at most that many lines (at a
+// level >= 1), each factorisation also forms the inverse Z of their reduced system
+// (about 20 m x m x m products at four lines), and every apply maps those lines'
+// right-hand sides to their solutions with one launch instead of running the
+// forward and backward levels above.  Fused factors only (tf_bcr_set_fused; an
+// unfused factor keeps the per-level sweeps).  0 (default): off.  Allocates Z;
+// from the next factorisation on.
+int tf_bcr_set_dense_top(tf_bcr* h, int lines) {
+  if (!h) return tf::fail("null handle");
+  if (lines != 0 && (lines < 2 || lines > TOPD_MAX)) return tf::fail("tf_bcr_set_dense_top: lines must be 0 or 2..4");
+  drop_graphs(h);
+  for (void* p : {h->Z, h->zpool}) (void)hipFree(p);
+  h->Z = h->zpool = nullptr;
+  h->zsteps.clear();
+  h->zline.clear();
+  h->zn = 0;
+  h->zl = 0;
+  h->zreplaced = 0;
+  h->z_ok = 0;
+  h->zmax = lines;
+  if (!lines) return 0;
+  // the lines active at level l are the kept lines of level l - 1
+  int zl = -1;
+  for (size_t l = 1; l < h->plan.size(); ++l)
+    if ((int)h->plan[l - 1].k.size() <= lines) {
+      zl = (int)l;
+      break;
+    }
+  if (zl < 0 || h->plan[zl - 1].k.size() < 2) return 0;  // only the last line: nothing to replace
+  const std::vector<int> A = h->plan[zl - 1].k;
+  const int zn = (int)A.size();
+  const int m = h->m;
+  const size_t mm = (size_t)m * m, bm = h->es * mm;
+  std::vector<int> pos(h->N, -1);
+  for (int i = 0; i < zn; ++i) pos[A[i]] = i;
+  void *Z = nullptr, *pool = nullptr;
+  auto release = [&](hipError_t e, const char* what) {
+    for (void* p : {Z, pool}) (void)hipFree(p);
+    return tf::fail_hip(e, what);
+  };
+  hipError_t e = hipMalloc(&Z, bm * zn * zn);
+  if (e != hipSuccess) return release(e, "hipMalloc (dense top)");
+  auto zb = [&](int li, int lj) -> void* { return (char*)Z + bm * (size_t)(pos[li] * zn + pos[lj]); };
+  auto fb = [&](void* base, int j) -> void* { return (char*)base + bm * (size_t)j; };
+  // the steps, from the last level down to zl: after level l, Z[i][j] holds the
+  // inverse of the reduced system of the lines active at level l
+  struct HStep {
+    int kind;
+    std::vector<void*> a, b, c;
+    double alpha, beta;
+  };
+  std::vector<HStep> hs;
+  long replaced = 0;
+  for (int l = (int)h->plan.size() - 1; l >= zl; --l) {
+    const LevelPlan& L = h->plan[l];
+    std::vector<int> act;  // the lines active at level l
+    act.insert(act.end(), L.e.begin(), L.e.end());
+    act.insert(act.end(), L.k.begin(), L.k.end());
+    // kept rows, eliminated columns: Z[k][e] = -sum Z[k][k'] P1_k' (e = e1 of k') - sum Z[k][k'] P2_k' (e = e2 of k')
+    if (!L.k.empty() && !L.e.empty()) {
+      HStep z{0, {}, {}, {}, 0.0, 0.0}, g1{2, {}, {}, {}, -1.0, 1.0}, g2{2, {}, {}, {}, -1.0, 1.0};
+      for (int k : L.k)
+        for (int ee : L.e) z.a.push_back(zb(k, ee));
+      for (size_t b = 0; b < L.k.size(); ++b)
+        for (int k : L.k) {
+          if (L.k1[b] >= 0) {
+            g1.a.push_back(zb(k, L.k[b]));
+            g1.b.push_back(fb(h->P1, L.kslot[b]));
+            g1.c.push_back(zb(k, L.k1[b]));
+          }
+          if (L.k2[b] >= 0) {
+            g2.a.push_back(zb(k, L.k[b]));
+            g2.b.push_back(fb(h->P2, L.kslot[b]));
+            g2.c.push_back(zb(k, L.k2[b]));
+          }
+        }
+      for (HStep* s : {&z, &g1, &g2})
+        if (!s->a.empty()) hs.push_back(*s);
+    }
+    // eliminated rows: Z[e][j] = Y_e [e == j] - (Y_e L_e) Z[a][j] - (Y_e U_e) Z[b][j]
+    HStep cp{1, {}, {}, {}, 0.0, 0.0}, z{0, {}, {}, {}, 0.0, 0.0}, gl{2, {}, {}, {}, -1.0, 1.0},
+        gu{2, {}, {}, {}, -1.0, 1.0};
+    for (size_t b = 0; b < L.e.size(); ++b) {
+      const int ee = L.e[b];
+      cp.a.push_back(fb(h->D, ee));
+      cp.b.push_back(zb(ee, ee));
+      for (int j : act) {
+        if (j != ee) z.a.push_back(zb(ee, j));
+        if (L.ea[b] >= 0) {
+          gl.a.push_back(fb(h->Lm, ee));
+          gl.b.push_back(zb(L.ea[b], j));
+          gl.c.push_back(zb(ee, j));
+        }
+        if (L.eb[b] >= 0) {
+          gu.a.push_back(fb(h->Um, ee));
+          gu.b.push_back(zb(L.eb[b], j));
+          gu.c.push_back(zb(ee, j));
+        }
+      }
+      replaced += 1 + (L.ea[b] >= 0) + (L.eb[b] >= 0);
+    }
+    for (size_t b = 0; b < L.k.size(); ++b) replaced += (L.k1[b] >= 0) + (L.k2[b] >= 0);
+    for (HStep* s : {&cp, &z, &gl, &gu})
+      if (!s->a.empty()) hs.push_back(*s);
+  }
+  PoolBuilder pb;
+  std::vector<std::array<size_t, 3>> off;
+  for (const HStep& s : hs) off.push_back({pb.add_ptrs(s.a), pb.add_ptrs(s.b), pb.add_ptrs(s.c)});
+  pb.align();
+  e = hipMalloc(&pool, pb.bytes.size() + 64);
+  if (e == hipSuccess) e = tf::upload(pool, pb.bytes.data(), pb.bytes.size());
+  if (e != hipSuccess) return release(e, "dense top pointer lists");
+  char* P = (char*)pool;
+  for (size_t i = 0; i < hs.size(); ++i)
+    h->zsteps.push_back(BcrZStep{hs[i].kind, (int)hs[i].a.size(), (void**)(P + off[i][0]), (void**)(P + off[i][1]),
+                                 (void**)(P + off[i][2]), hs[i].alpha, hs[i].beta});
+  h->Z = Z;
+  h->zpool = pool;
+  h->zl = zl;
+  h->zn = zn;
+  h->zline = A;
+  h->zreplaced = replaced;
   return 0;
 }
 
