@@ -46,7 +46,8 @@ struct tf_krylov {
   double* h;   // m+2 (device)
   double* y;   // m+1 (device)
   double* work;
-  double* h_host;  // m+2 (pinned)
+  double* h_host;      // m+2 (pinned, coherent)
+  double* h_host_dev;  // h_host as kernels address it
   MatrixDesc* desc;          // the matrix of the current solve (device)
   hipStream_t cap;           // private stream the step graphs are captured on
   hipGraphExec_t step[MAX_STEPS];
@@ -61,16 +62,18 @@ struct tf_krylov {
 };
 
 namespace tf {
+// hh0, hh1: the device address of pinned host copies of h (or null); *to_host = 1
+// when the launches wrote the column there
 int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, double* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, double* v0, double* v1,
-                  hipStream_t st);
+                  double* hh0, double* hh1, int* to_host, hipStream_t st);
 int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
-                  hipStream_t st);
+                  double* hh0, double* hh1, int* to_host, hipStream_t st);
 int cgs2_one_f64(int n, int j, const double* V, double* w, double* work, double* h, double* y, double* v,
-                 hipStream_t st);
+                 double* hh, int* to_host, hipStream_t st);
 int cgs2_one_f32(int n, int j, const float* V, float* w, double* work, double* h, double* y, float* v,
-                 hipStream_t st);
+                 double* hh, int* to_host, hipStream_t st);
 }  // namespace tf
 
 namespace {
@@ -152,7 +155,8 @@ int norm(const tf_krylov* k, int n, const void* x, double* out, hipStream_t st) 
 
 // The device part of Arnoldi step j on stream st: w = A Z[j] (through the
 // descriptor), CGS2 h = V^T w, w -= V h, y = V^T w, w -= V y, h += y, V[j+1] =
-// w / |w| with |w| into h[j+1], and h[0..j+1] copied to h_host.
+// w / |w| with |w| into h[j+1], and h[0..j+1] into h_host (written there by the
+// step's last kernel, or copied).
 int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
   const int n = k->n;
   Ops o{k, n, st};
@@ -163,12 +167,13 @@ int arnoldi_step(tf_krylov* k, int j, hipStream_t st) {
     hipLaunchKernelGGL(k_spmv_desc<double>, dim3(tf::spmv_blocks<double>(n)), dim3(256), 0, st, n, k->desc,
                        (const double*)o.Z(j), (double*)k->w);
   TF_LAUNCH_CHECK("fgmres spmv");
+  int to_host = 0;
   if (int e = k->single ? tf::cgs2_one_f32(n, j, (const float*)k->V, (float*)k->w, k->work, k->h, k->y,
-                                           (float*)o.V(j + 1), st)
+                                           (float*)o.V(j + 1), k->h_host_dev, &to_host, st)
                         : tf::cgs2_one_f64(n, j, (const double*)k->V, (double*)k->w, k->work, k->h, k->y,
-                                           (double*)o.V(j + 1), st))
+                                           (double*)o.V(j + 1), k->h_host_dev, &to_host, st))
     return e;
-  TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  if (!to_host) TF_HIP(hipMemcpyAsync(k->h_host, k->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
   return 0;
 }
 
@@ -216,15 +221,19 @@ int arnoldi_step2(tf_krylov* a, tf_krylov* b, int j, hipStream_t st) {
     hipLaunchKernelGGL(k_spmv_desc2<double>, dim3(tf::spmv_blocks<double>(n), 2), dim3(256), 0, st, n, a->desc,
                        (const double*)oa.Z(j), (double*)a->w, b->desc, (const double*)ob.Z(j), (double*)b->w);
   TF_LAUNCH_CHECK("fgmres spmv pair");
+  int to_host = 0;
   if (int e = a->single ? tf::cgs2_pair_f32(n, j, (const float*)a->V, (const float*)b->V, (float*)a->w, (float*)b->w,
                                             a->work, b->work, a->h, b->h, a->y, b->y, (float*)oa.V(j + 1),
-                                            (float*)ob.V(j + 1), st)
+                                            (float*)ob.V(j + 1), a->h_host_dev, b->h_host_dev, &to_host, st)
                         : tf::cgs2_pair_f64(n, j, (const double*)a->V, (const double*)b->V, (double*)a->w,
                                             (double*)b->w, a->work, b->work, a->h, b->h, a->y, b->y,
-                                            (double*)oa.V(j + 1), (double*)ob.V(j + 1), st))
+                                            (double*)oa.V(j + 1), (double*)ob.V(j + 1), a->h_host_dev, b->h_host_dev,
+                                            &to_host, st))
     return e;
-  TF_HIP(hipMemcpyAsync(a->h_host, a->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
-  TF_HIP(hipMemcpyAsync(b->h_host, b->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  if (!to_host) {
+    TF_HIP(hipMemcpyAsync(a->h_host, a->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+    TF_HIP(hipMemcpyAsync(b->h_host, b->h, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+  }
   return 0;
 }
 
@@ -265,15 +274,6 @@ int arnoldi_step2_graph(tf_krylov* a, tf_krylov* b, int j, hipStream_t st) {
   }
   TF_HIP(hipGraphLaunch(a->step2[j], st));
   return 0;
-}
-
-// Wait for stream st by polling: returns once everything queued has finished.
-int poll_wait(hipStream_t st) {
-  for (;;) {
-    const hipError_t q = hipStreamQuery(st);
-    if (q == hipSuccess) return 0;
-    if (q != hipErrorNotReady) return tf::fail_hip(q, "hipStreamQuery");
-  }
 }
 
 // Wait for event ev by polling.
@@ -491,14 +491,12 @@ int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* 
       for (int q = 0; q < nc; ++q)
         if (c[q].stepping && !col_continues(c[q], j, maxit)) ahead = false;
       have_z = ahead;
-      if (ahead) {
-        hipEvent_t ev = c[0].k->step_done;
-        TF_HIP(hipEventRecord(ev, st));
+      // the event marks the step's Hessenberg columns as visible to the host
+      hipEvent_t ev = c[0].k->step_done;
+      TF_HIP(hipEventRecord(ev, st));
+      if (ahead)
         if (int e = precond_columns(c, nc, j + 1, prec_kind, prec, ncell, dof, st)) return e;
-        if (int e = poll_event(ev)) return e;
-      } else if (int e = poll_wait(st)) {
-        return e;
-      }
+      if (int e = poll_event(ev)) return e;
       for (int q = 0; q < nc; ++q)
         if (c[q].stepping) col_givens(c[q], j, maxit);
     }
@@ -531,7 +529,10 @@ int tf_krylov_create(tf_krylov** out, int n, int restart, int single) {
     TF_HIP(hipMalloc(&k->h, sizeof(double) * (restart + 2)));
     TF_HIP(hipMalloc(&k->y, sizeof(double) * (restart + 1)));
     TF_HIP(hipMalloc(&k->work, sizeof(double) * (size_t)tf_krylov_work_size()));
-    TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocDefault));
+    // coherent: the Arnoldi step's last kernel writes the Hessenberg column here
+    // and the host reads it as soon as the step's completion is seen
+    TF_HIP(hipHostMalloc(&k->h_host, sizeof(double) * (restart + 2), hipHostMallocCoherent));
+    TF_HIP(hipHostGetDevicePointer((void**)&k->h_host_dev, k->h_host, 0));
     TF_HIP(hipMalloc(&k->desc, sizeof(MatrixDesc)));
     TF_HIP(hipEventCreateWithFlags(&k->step_done, hipEventDisableTiming));
     return 0;

@@ -246,16 +246,20 @@ __global__ __launch_bounds__(256) void k_multiaxpy_fin(int n, int k, int nb, con
   axpy_body<T, VEC>(n, k, V, ldv, c, w);
 }
 
-// nrm = sqrt(sum of partial[0..nb)) (block 0 stores it), out = w / nrm
+// nrm = sqrt(sum of partial[0..nb)) (block 0 stores it in h[k]), out = w / nrm.
+// hhost (pinned host memory, or null): block 0 also writes the column h[0..k] there,
+// so the host reads it without a copy after the launch.
 template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void k_scale_fin(int n, int nb, const double* __restrict__ partial,
-                                                   double* __restrict__ nrm_out, const T* __restrict__ w,
-                                                   T* __restrict__ out, const double* __restrict__ partial1,
-                                                   double* __restrict__ nrm_out1, const T* __restrict__ w1,
+__global__ __launch_bounds__(256) void k_scale_fin(int n, int nb, int k, const double* __restrict__ partial,
+                                                   double* __restrict__ h, double* __restrict__ hhost,
+                                                   const T* __restrict__ w, T* __restrict__ out,
+                                                   const double* __restrict__ partial1, double* __restrict__ h1,
+                                                   double* __restrict__ hhost1, const T* __restrict__ w1,
                                                    T* __restrict__ out1) {
   if (blockIdx.y) {
     partial = partial1;
-    nrm_out = nrm_out1;
+    h = h1;
+    hhost = hhost1;
     w = w1;
     out = out1;
   }
@@ -263,7 +267,14 @@ __global__ __launch_bounds__(256) void k_scale_fin(int n, int nb, const double* 
   __shared__ double c[1];
   fin_sums(1, nb, partial, sp, c);
   const double nrm = sqrt(c[0]);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm_out = nrm;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) h[k] = nrm;
+    if (hhost) {
+      // h[0..k) is final since the previous launch
+      for (int i = threadIdx.x; i < k; i += blockDim.x) hhost[i] = h[i];
+      if (threadIdx.x == 0) hhost[k] = nrm;
+    }
+  }
   using VT = typename Vec16<T>::type;
   constexpr int VN = VEC ? Vec16<T>::N : 1;
   const double inv = 1.0 / nrm;
@@ -508,11 +519,15 @@ __global__ void k_add_pair(int k, double* h0, const double* y0, double* h1, cons
 // the sums are reduced by the consuming kernels (k_multiaxpy_fin, k_scale_fin:
 // five launches); else by separate kernels (k_multidot_final, k_add_pair,
 // k_norm_final: eight).  The choice depends on n and j only, so one system and
-// two systems take the same arithmetic.
+// two systems take the same arithmetic.  hh0, hh1 (pinned host memory, or null):
+// on the five-launch path the last launch also writes each column h[0..j+1] there
+// (*to_host = 1); otherwise *to_host = 0 and the caller copies it.
 template <typename T>
 int cgs2_sys(int nsys, int n, int j, const T* V0, const T* V1, T* w0, T* w1, double* work0, double* work1,
-             double* h0, double* h1, double* y0, double* y1, T* v0, T* v1, hipStream_t st) {
+             double* h0, double* h1, double* y0, double* y1, T* v0, T* v1, double* hh0, double* hh1, int* to_host,
+             hipStream_t st) {
   const int k = j + 1;
+  *to_host = 0;
   if (k < 1 || k + 1 > MAX_BASIS) return tf::fail("cgs2: bad basis size");
   if (nsys == 1) {
     V1 = V0;
@@ -521,6 +536,7 @@ int cgs2_sys(int nsys, int n, int j, const T* V0, const T* V1, T* w0, T* w1, dou
     h1 = h0;
     y1 = y0;
     v1 = v0;
+    hh1 = hh0;
   }
   const int nb = md_blocks(n);
   const bool vec = vec_ok<T>(n, n, V0, w0) && vec_ok<T>(n, n, V1, w1) && vec_ok<T>(n, n, v0, v1);
@@ -565,12 +581,13 @@ int cgs2_sys(int nsys, int n, int j, const T* V0, const T* V1, T* w0, T* w1, dou
                        work0, (const T*)w1, (const T*)w1, work1);
   if (fused) {
     if (vec)
-      hipLaunchKernelGGL((k_scale_fin<T, true>), ew, dim3(256), 0, st, n, nb, (const double*)work0, h0 + k,
-                         (const T*)w0, v0, (const double*)work1, h1 + k, (const T*)w1, v1);
+      hipLaunchKernelGGL((k_scale_fin<T, true>), ew, dim3(256), 0, st, n, nb, k, (const double*)work0, h0, hh0,
+                         (const T*)w0, v0, (const double*)work1, h1, hh1, (const T*)w1, v1);
     else
-      hipLaunchKernelGGL((k_scale_fin<T, false>), ew, dim3(256), 0, st, n, nb, (const double*)work0, h0 + k,
-                         (const T*)w0, v0, (const double*)work1, h1 + k, (const T*)w1, v1);
+      hipLaunchKernelGGL((k_scale_fin<T, false>), ew, dim3(256), 0, st, n, nb, k, (const double*)work0, h0, hh0,
+                         (const T*)w0, v0, (const double*)work1, h1, hh1, (const T*)w1, v1);
     TF_LAUNCH_CHECK("cgs2");
+    *to_host = hh0 != nullptr && hh1 != nullptr;
     return 0;
   }
   hipLaunchKernelGGL(k_norm_final, dim3(1, nsys), blk, 0, st, nb, (const double*)work0, h0 + k, (const double*)work1,
@@ -592,22 +609,24 @@ namespace tf {
 // in the same launches, or for one; the same arithmetic either way.
 int cgs2_pair_f64(int n, int j, const double* V0, const double* V1, double* w0, double* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, double* v0, double* v1,
-                  hipStream_t st) {
-  return cgs2_sys<double>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+                  double* hh0, double* hh1, int* to_host, hipStream_t st) {
+  return cgs2_sys<double>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, hh0, hh1, to_host, st);
 }
 int cgs2_pair_f32(int n, int j, const float* V0, const float* V1, float* w0, float* w1, double* work0,
                   double* work1, double* h0, double* h1, double* y0, double* y1, float* v0, float* v1,
-                  hipStream_t st) {
-  return cgs2_sys<float>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, st);
+                  double* hh0, double* hh1, int* to_host, hipStream_t st) {
+  return cgs2_sys<float>(2, n, j, V0, V1, w0, w1, work0, work1, h0, h1, y0, y1, v0, v1, hh0, hh1, to_host, st);
 }
 // the same for one system
 int cgs2_one_f64(int n, int j, const double* V, double* w, double* work, double* h, double* y, double* v,
-                 hipStream_t st) {
-  return cgs2_sys<double>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, st);
+                 double* hh, int* to_host, hipStream_t st) {
+  return cgs2_sys<double>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, hh,
+                          nullptr, to_host, st);
 }
 int cgs2_one_f32(int n, int j, const float* V, float* w, double* work, double* h, double* y, float* v,
-                 hipStream_t st) {
-  return cgs2_sys<float>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, st);
+                 double* hh, int* to_host, hipStream_t st) {
+  return cgs2_sys<float>(1, n, j, V, nullptr, w, nullptr, work, nullptr, h, nullptr, y, nullptr, v, nullptr, hh,
+                         nullptr, to_host, st);
 }
 }  // namespace tf
 
