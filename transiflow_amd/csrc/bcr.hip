@@ -961,9 +961,15 @@ struct Cols {
   P p[NC];
 };
 
-// rw (line-major, NC columns at stride cs) <- the right-hand sides r[j] in natural order
+// rw (line-major, NC columns at stride cs) <- the right-hand sides r[j] in natural order;
+// xtab (or null) <- the output pointers x[j] for the replayed graph's k_stage_out_tab
 template <typename V, int NC>
-__global__ __launch_bounds__(256) void k_stage_in(int n, Order o, Cols<const V*, NC> r, V* __restrict__ rw, long cs) {
+__global__ __launch_bounds__(256) void k_stage_in(int n, Order o, Cols<const V*, NC> r, V* __restrict__ rw, long cs,
+                                                  Cols<V*, NC> x, V** __restrict__ xtab) {
+  if (xtab && blockIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      if (threadIdx.x == j) xtab[j] = x.p[j];
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const int s = o.to_natural(q);
 #pragma unroll
@@ -978,6 +984,21 @@ __global__ __launch_bounds__(256) void k_stage_out(int n, Order o, const V* __re
     const int s = o.to_natural(q);
 #pragma unroll
     for (int j = 0; j < NC; ++j) x.p[j][s] = xo[j * cs + q];
+  }
+}
+
+// the same with the output pointers read from xtab (written by this apply's k_stage_in),
+// so that the captured graph ends with the scatter whatever the caller's x
+template <typename V, int NC>
+__global__ __launch_bounds__(256) void k_stage_out_tab(int n, Order o, const V* __restrict__ xo, long cs,
+                                                       V* const* __restrict__ xtab) {
+  V* x[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) x[j] = xtab[j];
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int s = o.to_natural(q);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j][s] = xo[j * cs + q];
   }
 }
 
@@ -1067,6 +1088,7 @@ struct tf_bcr {
   int n, m, N, single;
   Order order;
   void* xw;    // 2n (line-major solution when the order is not natural; two columns)
+  void* xtab;  // 2 pointers: the outputs of the current graph apply (k_stage_in -> k_stage_out_tab)
   // captured apply: the sweeps rw -> xw replayed as one graph launch (per vector type and column count)
   int use_graph;
   hipStream_t cap;
@@ -1541,12 +1563,18 @@ void run_levels(tf_bcr* h, V* rw, V* xo, hipStream_t st) {
   if (!half) levels<T, V, NC>(h, rw, tb, xo, n, st);
 }
 
-// r[j] (natural order) -> the work vector rw, one launch for all columns
+// r[j] (natural order) -> the work vector rw, one launch for all columns; x (or null):
+// the output pointers into the handle's table for the graph's scatter
 template <typename V, int NC>
-void stage_in(const tf_bcr* h, const V* const (&r)[NC], V* rw, hipStream_t st) {
+void stage_in(const tf_bcr* h, const V* const (&r)[NC], V* rw, V* const (*x)[NC], hipStream_t st) {
   Cols<const V*, NC> c;
-  for (int j = 0; j < NC; ++j) c.p[j] = r[j];
-  hipLaunchKernelGGL((k_stage_in<V, NC>), dim3(1024), dim3(256), 0, st, h->n, h->order, c, rw, (long)h->n);
+  Cols<V*, NC> xc{};
+  for (int j = 0; j < NC; ++j) {
+    c.p[j] = r[j];
+    if (x) xc.p[j] = (*x)[j];
+  }
+  hipLaunchKernelGGL((k_stage_in<V, NC>), dim3(1024), dim3(256), 0, st, h->n, h->order, c, rw, (long)h->n, xc,
+                     x ? (V**)h->xtab : (V**)nullptr);
 }
 
 // xo (line-major) -> x[j] (natural order), one launch for all columns
@@ -1566,7 +1594,7 @@ int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_
   // one column in natural order: the kernels write the solution in place
   const bool direct = !h->order.ylines && NC == 1;
   V* xo = direct ? x[0] : (V*)h->xw;
-  stage_in<V, NC>(h, r, rw, st);
+  stage_in<V, NC>(h, r, rw, nullptr, st);
   run_levels<T, V, NC>(h, rw, xo, st);
   if (!direct) stage_out<V, NC>(h, xo, x, st);
   TF_PROF_END(solve_prof_id(NC), st);
@@ -1574,10 +1602,12 @@ int solve_impl(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_
   return 0;
 }
 
-// The sweeps (rw -> xw) captured once per (vector type, column count) on a private
-// stream and replayed with one graph launch between the two staging kernels; the
-// factor refactors in place, so the captured pointers stay valid.  The graph holds
-// kernel nodes only.
+// The sweeps (rw -> xw) and the scatter to the caller's x captured once per (vector
+// type, column count) on a private stream and replayed with one graph launch after
+// the gather kernel, which also writes this call's x pointers into the table the
+// scatter reads (a scatter launched after the replay started about 8 us after its
+// end).  The factor refactors in place, so the captured pointers stay valid.  The
+// graph holds kernel nodes only.
 template <typename T, typename V, int NC>
 int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream_t st) {
   BcrGraph& gs = (sizeof(V) == 8 ? h->g64 : h->g32)[NC - 1];
@@ -1594,6 +1624,8 @@ int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream
       std::lock_guard<std::mutex> lock(tf::capture_mutex());
       TF_HIP(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
       run_levels<T, V, NC>(h, rw, xw, h->cap);
+      hipLaunchKernelGGL((k_stage_out_tab<V, NC>), dim3(1024), dim3(256), 0, h->cap, h->n, h->order,
+                         (const V*)xw, (long)h->n, (V* const*)h->xtab);
       le = hipGetLastError();
       ce = hipStreamEndCapture(h->cap, &g);
       if (le == hipSuccess && ce == hipSuccess && g) ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -1612,9 +1644,8 @@ int solve_graph(tf_bcr* h, const V* const (&r)[NC], V* const (&x)[NC], hipStream
   // the profiler's events bracket what an apply costs on this path: the staging kernels and the replay
   TF_PROF_WORK(solve_prof_id(NC), (solve_work<T, V, NC>(h)));
   TF_PROF_BEGIN(solve_prof_id(NC), st);
-  stage_in<V, NC>(h, r, rw, st);
+  stage_in<V, NC>(h, r, rw, &x, st);
   TF_HIP(hipGraphLaunch(gs.exec, st));
-  stage_out<V, NC>(h, xw, x, st);
   TF_PROF_END(solve_prof_id(NC), st);
   TF_LAUNCH_CHECK("bcr graph apply");
   return 0;
@@ -1714,6 +1745,7 @@ static int bcr_create_impl(tf_bcr* h, int n, int m, int single, int nx, int dof,
   TF_HIP(hipMalloc(&h->tb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->yb, 2 * sizeof(double) * (size_t)n));   // two columns
   TF_HIP(hipMalloc(&h->xw, 2 * sizeof(double) * (size_t)n));   // two columns
+  TF_HIP(hipMalloc(&h->xtab, 2 * sizeof(void*)));
   TF_HIP(hipMalloc(&h->topsync, sizeof(unsigned) * TOP_SYNC_WORDS));
   {
     // zeroed through the library's non-blocking upload stream: a hipMemset on the legacy
@@ -1965,7 +1997,7 @@ int tf_bcr_destroy(tf_bcr* h) {
   for (void* p : {h->D, h->Lm, h->Um, h->P1, h->P2, h->Pinv, h->Kr, h->Kc, h->Tb, (void*)h->Lr, (void*)h->Ur,
                   (void*)h->Ut, (void*)h->Lt, (void*)h->nLr, (void*)h->nUr, (void*)h->nUt, (void*)h->nLt,
                   (void*)h->flags, h->rw, h->tb, h->yb, h->xw, h->pool, h->H, h->S, h->hpool, h->W,
-                  (void*)h->topsync, h->Z, h->zpool})
+                  (void*)h->topsync, h->Z, h->zpool, h->xtab})
     (void)hipFree(p);
   (void)hipHostFree(h->host_flags);
   delete h;
