@@ -86,8 +86,11 @@ __device__ __forceinline__ void spmv_stream(int n, const int* __restrict__ rp, c
 
 // The launched form per value type, as measured on the cavity Jacobians:
 // FP32 values (the inner solves) stream (1024^2: 3.9 TB/s against 2.2 with 8
-// lanes per row); FP64 values take 8 lanes per row (2.7 TB/s against 2.4).
-constexpr int SPMV_LANES = 8;
+// lanes per row); FP64 values take 4 lanes per row (algorithmic bytes / time:
+// 2D 1024^2 3.4 TB/s, 256^2 3.2 TB/s; 8 lanes 2.7 and 2.6, 16 lanes 2.0 and 1.9,
+// streaming 2.4; 3D 64^3, 14 entries per row, 3.7 TB/s against 3.9 with 8
+// lanes).  A larger grid cap (32768 blocks) was slower at 1024^2 (2.5 TB/s).
+constexpr int SPMV_LANES = 4;
 constexpr int SPMV_MAX_BLOCKS = 8192;
 
 template <typename TV, typename TX>
