@@ -139,17 +139,21 @@ int precond(const tf_krylov* k, int kind, void* handle, int ncell, int dof, cons
   }
 }
 
+// |x|^2 queued into the handle's pinned word h_host[m + 1]: the reduction's last
+// kernel writes it there (no copy); read it after the stream is synchronised
+int norm_queue(const tf_krylov* k, int n, const void* x, hipStream_t st) {
+  double* out = k->h_host_dev + k->m + 1;
+  return k->single ? tf_multidot_f32(n, 1, (const float*)x, n, (const float*)x, k->work, out, 0, st)
+                   : tf_multidot_f64(n, 1, (const double*)x, n, (const double*)x, k->work, out, 0, st);
+}
+
+double norm_read(const tf_krylov* k) { return std::sqrt(k->h_host[k->m + 1]); }
+
 // ||x|| (host double; synchronises the stream)
 int norm(const tf_krylov* k, int n, const void* x, double* out, hipStream_t st) {
-  if (k->single) {
-    if (int e = tf_multidot_f32(n, 1, (const float*)x, n, (const float*)x, k->work, k->h + k->m + 1, 0, st)) return e;
-  } else {
-    if (int e = tf_multidot_f64(n, 1, (const double*)x, n, (const double*)x, k->work, k->h + k->m + 1, 0, st))
-      return e;
-  }
-  TF_HIP(hipMemcpyAsync(k->h_host + k->m + 1, k->h + k->m + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  if (int e = norm_queue(k, n, x, st)) return e;
   TF_HIP(hipStreamSynchronize(st));
-  *out = std::sqrt(k->h_host[k->m + 1]);
+  *out = norm_read(k);
   return 0;
 }
 
@@ -300,21 +304,34 @@ struct Column {
   double& Hij(int i, int j) { return H[(size_t)i * k->m + j]; }
 };
 
-// b's norm; a zero right-hand side has the solution zero
-int col_start(Column& c, hipStream_t st) {
-  const int n = c.k->n, m = c.k->m;
-  c.H.assign((size_t)(m + 1) * m, 0.0);
-  c.cs.assign(m, 0.0);
-  c.sn.assign(m, 0.0);
-  c.g.assign(m + 1, 0.0);
-  c.yk.assign(m, 0.0);
-  if (int e = norm(c.k, n, c.b, &c.bnrm, st)) return e;
+// bnrm = |b|; a zero right-hand side has the solution zero (done)
+int col_zero_check(Column& c, double bnrm, hipStream_t st) {
+  const int n = c.k->n;
+  c.bnrm = bnrm;
   c.rel = 0.0;
   if (c.bnrm == 0.0) {
     c.done = true;
     return c.k->single ? tf_fill_f32(n, 0.0, (float*)c.x, st) : tf_fill_f64(n, 0.0, (double*)c.x, st);
   }
   c.rel = INFINITY;
+  return 0;
+}
+
+// The columns' state and |b| (the norms of all columns queued, then one wait)
+int col_start(Column* cols, int nc, hipStream_t st) {
+  for (int q = 0; q < nc; ++q) {
+    Column& c = cols[q];
+    const int m = c.k->m;
+    c.H.assign((size_t)(m + 1) * m, 0.0);
+    c.cs.assign(m, 0.0);
+    c.sn.assign(m, 0.0);
+    c.g.assign(m + 1, 0.0);
+    c.yk.assign(m, 0.0);
+    if (int e = norm_queue(c.k, c.k->n, c.b, st)) return e;
+  }
+  TF_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < nc; ++q)
+    if (int e = col_zero_check(cols[q], norm_read(cols[q].k), st)) return e;
   return 0;
 }
 
@@ -446,8 +463,7 @@ int precond_columns(Column* c, int nc, int j, int kind, void* prec, int ncell, i
 
 int fgmres_columns(Column* c, int nc, const int* rp, const int* ci, const void* val, int gauge, int prec_kind,
                    void* prec, int ncell, int dof, int maxit, int x_is_zero, hipStream_t st) {
-  for (int q = 0; q < nc; ++q)
-    if (int e = col_start(c[q], st)) return e;
+  if (int e = col_start(c, nc, st)) return e;
   // the Arnoldi step graphs read the matrix through each handle's descriptor
   for (int q = 0; q < nc; ++q)
     hipLaunchKernelGGL(k_set_desc, dim3(1), dim3(1), 0, st, c[q].k->desc, rp, ci, val, gauge);
