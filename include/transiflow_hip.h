@@ -224,9 +224,10 @@ int tf_bcr_solve_f32(tf_bcr* h, const float* r, float* x, void* stream);
  * Newton-corrector iteration (Continuation.py:158-159: F_x z1 = -F, F_x z2 = F_mu). */
 int tf_bcr_solve2_f64(tf_bcr* h, const double* r0, const double* r1, double* x0, double* x1, void* stream);
 int tf_bcr_solve2_f32(tf_bcr* h, const float* r0, const float* r1, float* x0, float* x1, void* stream);
-/* on != 0: the level sweeps of the apply are captured once (per vector type and column
- * count) into a hipGraph and replayed with one launch between two staging kernels (no
- * reference counterpart; launch overhead on small grids and batches). */
+/* on != 0: the level sweeps of the apply and the scatter of its result are captured once
+ * (per vector type and column count) into a hipGraph and replayed with one launch after the
+ * gather kernel, which writes the call's output pointers into a device table the scatter
+ * reads (no reference counterpart; launch overhead on small grids and batches). */
 int tf_bcr_set_graph(tf_bcr* h, int on);
 /* on != 0 would run the levels with fewer than 12288 rows per launch as one ordered-task
  * launch (no reference counterpart; the launch floor of the upper levels); it returns an
