@@ -944,14 +944,35 @@ __global__ __launch_bounds__(256) void k_to_half(int m, float* const* __restrict
   if (r >= m) return;  // the whole wave leaves together
   const float* a = src[b] + (long)r * m;
   f16* o = dst[b] + (long)r * m;
+  // rows up to 64 * TH_REG entries are read once, into registers; longer rows twice
+  constexpr int TH_REG = 16;
+  const bool held = m <= 64 * TH_REG;
+  float v[TH_REG];
   float mx = 0.0f;
-  for (int c = lane; c < m; c += 64) mx = fmaxf(mx, fabsf(a[c]));
+  if (held) {
+#pragma unroll
+    for (int t = 0; t < TH_REG; ++t) {
+      const int c = lane + 64 * t;
+      v[t] = c < m ? a[c] : 0.0f;
+      mx = fmaxf(mx, fabsf(v[t]));
+    }
+  } else {
+    for (int c = lane; c < m; c += 64) mx = fmaxf(mx, fabsf(a[c]));
+  }
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
   int e = 0;
   if (mx > 0.0f && mx <= 3.0e38f) (void)frexpf(mx, &e);  // mx < 2^e
   if (e < -100) e = -100;
   const float inv = ldexpf(1.0f, 15 - e);
-  for (int c = lane; c < m; c += 64) o[c] = (f16)(a[c] * inv);
+  if (held) {
+#pragma unroll
+    for (int t = 0; t < TH_REG; ++t) {
+      const int c = lane + 64 * t;
+      if (c < m) o[c] = (f16)(v[t] * inv);
+    }
+  } else {
+    for (int c = lane; c < m; c += 64) o[c] = (f16)(a[c] * inv);
+  }
   if (lane == 0) scl[b][r] = ldexpf(1.0f, e - 15);
 }
 
