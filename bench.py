@@ -471,6 +471,36 @@ def assembly_sample(reps=3):
     log('assembly sample done')
 
 
+def krylov_sample(reps=3):
+    """Short launch sequence for hardware-counter runs of the Krylov kernels at sizes
+    that stream from HBM: on the 1024 x 1024 and 64^3 cavity Jacobians at Re = 1000
+    (random state), ``reps`` rounds of vector_rooflines' launches (FP64 SpMV, FP32
+    SpMV, one FP32 CGS2 Arnoldi step's vector work at basis 16), then the plain copy
+    of kernel_sample as calibration."""
+    import numpy
+    import torch
+    import transiflow_amd as tf
+    from transiflow_amd import _lib
+    from transiflow_amd.vector import ptr, stream
+    for shape in ((1024, 1024, 1), (64, 64, 64)):
+        itf = tf.Interface({'Reynolds Number': 1000.0}, *shape)
+        x = itf.vector_from_array(numpy.random.default_rng(3).uniform(-0.1, 0.1, itf.n))
+        r = vector_rooflines(torch, itf, x, reps=reps)
+        torch.cuda.synchronize()
+        log('krylov sample %s: n %d, nnz %d, SpMV FP64 %.1f us, FP32 %.1f us, Arnoldi %.1f us'
+            % ('x'.join(map(str, shape)), r['n'], r['nnz'], r['spmv_f64']['us'], r['spmv_f32']['us'],
+               r['gmres_arnoldi_vector_ops_f32']['us']))
+        del itf, x
+        torch.cuda.empty_cache()
+    big = 64 * 2 ** 20
+    a = torch.ones(big, dtype=torch.float32, device='cuda')
+    b = torch.ones(big, dtype=torch.float32, device='cuda')
+    c = torch.empty(big, dtype=torch.float32, device='cuda')
+    _lib.call('tf_axpby_f32', big, 1.0, ptr(a), -1.0, ptr(b), ptr(c), stream())
+    torch.cuda.synchronize()
+    log('krylov sample done')
+
+
 def conv_sample(reps=5, n=64):
     """Short launch sequence for the learned network's MFMA kernels: the 3D
     cavity's network (4 -> 32 -> 32 -> 4 channels, 3x3x3) on an n^3 grid,
@@ -567,6 +597,9 @@ def main():
                     help='run only the short launch sequence used for hardware-counter profiles')
     ap.add_argument('--assembly-sample', action='store_true',
                     help='only the assembly launch sequence for hardware-counter runs (1024^2 and 64^3)')
+    ap.add_argument('--krylov-sample', action='store_true',
+                    help='only the SpMV and Arnoldi-vector launch sequence for hardware-counter runs '
+                         '(1024^2 and 64^3)')
     ap.add_argument('--conv-sample', action='store_true',
                     help="run only the learned network's forward and backward passes at 64^3 (MFMA profiles)")
     args = ap.parse_args()
@@ -583,6 +616,9 @@ def main():
         return
     if args.assembly_sample:
         assembly_sample()
+        return
+    if args.krylov_sample:
+        krylov_sample()
         return
     if args.workload == 'batch':
         batch_main(args)
