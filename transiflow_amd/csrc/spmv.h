@@ -14,6 +14,15 @@
 
 namespace tf {
 
+// XCD-aware block order: the hardware deals workgroups to the eight XCDs (each with its
+// own L2) round-robin, so block b runs as logical block (b % 8) (nb / 8) + b / 8 and each
+// XCD streams contiguous row ranges, fetching into its L2 only the part of x those rows
+// gather.  A bijection on [0, nb) (the identity when nb is not a multiple of 8); the
+// rows' arithmetic does not depend on which block runs them.  Measured against the plain
+// order: 256^2 FP64 6.1 against 7.7 us, FP32 5.3 against 6.3; 64^3 49.9 against 52.6 and
+// 32.8 against 33.6; 1024^2 within 2% either way.
+__device__ __forceinline__ int xcd_block(int b, int nb) { return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3); }
+
 template <int G, typename TV, typename TX>
 __device__ __forceinline__ void spmv_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                           const TV* __restrict__ v, const TX* __restrict__ x, TX* __restrict__ y,
@@ -24,7 +33,8 @@ __device__ __forceinline__ void spmv_rows(int n, const int* __restrict__ rp, con
   using Acc = typename std::conditional<sizeof(TX) == 8, double, float>::type;
   const int lane = threadIdx.x & (G - 1);
   const long stride = (long)gridDim.x * blockDim.x / G;
-  for (long row = (blockIdx.x * (long)blockDim.x + threadIdx.x) / G; row < n; row += stride) {
+  const long b = xcd_block(blockIdx.x, gridDim.x);
+  for (long row = (b * blockDim.x + threadIdx.x) / G; row < n; row += stride) {
     Acc acc = 0;
     if (row != gauge) {
       const int e = rp[row + 1];
@@ -55,7 +65,7 @@ __device__ __forceinline__ void spmv_stream(int n, const int* __restrict__ rp, c
   using Acc = typename std::conditional<sizeof(TX) == 8, double, float>::type;
   __shared__ Acc prod[SPMV_CAP];
   __shared__ int rps[SPMV_ROWS + 1];
-  const int r0 = blockIdx.x * SPMV_ROWS;
+  const int r0 = xcd_block(blockIdx.x, gridDim.x) * SPMV_ROWS;
   const int nr = min(SPMV_ROWS, n - r0);
   for (int t = threadIdx.x; t <= nr; t += blockDim.x) rps[t] = rp[r0 + t];
   __syncthreads();
