@@ -10,8 +10,8 @@ correction of the iterate:
 
 ``alpha`` is the least-squares step along the correction, so the corrected
 residual r1 - alpha A c is never larger than r1: the base preconditioner is
-both the fallback (alpha <= 0 or a negligible gain keeps z = z0) and the judge
-of the network's error (the residual it measures decides).  FGMRES accepts the
+both the fallback (alpha <= 0 or a negligible gain keeps z = z0) and the
+measure of the network's error (the residual it leaves decides).  FGMRES accepts the
 resulting non-linear preconditioner.
 
 Network: dof -> 32 -> 32 -> dof channels, 3x3 (2D) or 3x3x3 (3D) stencils on
