@@ -58,6 +58,7 @@ struct tf_krylov {
   unsigned long long uid;            // unique per handle (a freed handle's address can come back)
   unsigned long long step2_partner;  // uid of the partner the step2 graphs were captured with
   hipEvent_t step_done;              // recorded after a step's Hessenberg column is queued to the host
+  double last_sn0;                   // |sine| of the first rotation of the last cycle (0: none yet)
   int use_graph;
 };
 
@@ -396,6 +397,7 @@ void col_givens(Column& c, int j, int maxit) {
   c.Hij(j + 1, j) = 0.0;
   c.g[j + 1] = -c.sn[j] * c.g[j];
   c.g[j] = c.cs[j] * c.g[j];
+  if (j == 0) c.k->last_sn0 = std::fabs(c.sn[0]);
   ++c.its;
   c.kk = j + 1;
   if (std::fabs(c.g[j + 1]) / c.bnrm <= tol || c.its >= maxit || !std::isfinite(c.g[j + 1])) c.stepping = false;
@@ -403,12 +405,13 @@ void col_givens(Column& c, int j, int maxit) {
 
 // Whether a stepping column will most likely still be stepping after step j: the
 // iteration limit is not reached and its residual estimate |g_j| times the last
-// reduction factor |s_{j-1}| (the sine of the previous rotation) stays above four
-// times the tolerance.
+// reduction factor |s_{j-1}| (the sine of the previous rotation; at j = 0 the first
+// rotation's of the handle's previous cycle) stays above four times the tolerance.
 bool col_continues(const Column& c, int j, int maxit) {
-  if (j < 1 || c.its + 1 >= maxit) return false;
-  const double pred = std::fabs(c.g[j]) * std::fabs(c.sn[j - 1]);
-  return std::isfinite(pred) && pred > 4.0 * c.tol * c.bnrm;
+  if (c.its + 1 >= maxit) return false;
+  const double s = j >= 1 ? std::fabs(c.sn[j - 1]) : c.k->last_sn0;
+  const double pred = std::fabs(c.g[j]) * s;
+  return s > 0.0 && std::isfinite(pred) && pred > 4.0 * c.tol * c.bnrm;
 }
 
 // x += Z y with H y = g (upper triangular); a single-precision solve that ran out
