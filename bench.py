@@ -295,8 +295,9 @@ def assembly_roofline(tf, torch, itf, x, reps=20):
 PROFILED = {
     5: ('k_gemm_mfma', 'mfma', 'flop'),
     3: ('k_panel', 'mfma', 'flop'),
-    9: ('line-factor apply (k_stage_in, k_fwd, k_bwd_fused, k_bwd_t, k_bwd_x, k_stage_out)', 'hbm', 'byte'),
-    10: ('two-column line-factor apply (k_stage_in, k_fwd, k_bwd_fused, k_bwd_t, k_bwd_x, k_stage_out; NC=2)',
+    9: ('line-factor apply (k_stage_in, k_bwd_x, k_fwd_sparse, k_fwd, k_small, k_bwd_fused, k_bwd_t_sparse, k_stage_out_tab)',
+        'hbm', 'byte'),
+    10: ('two-column line-factor apply (k_stage_in, k_bwd_x, k_fwd_sparse, k_fwd, k_small, k_bwd_fused, k_bwd_t_sparse, k_stage_out_tab; NC=2)',
          'hbm', 'byte'),
 }
 
